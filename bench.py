@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the closest-point / signed-distance hot path on one MI355X.
+
+Workload (default): noisy torus, R=0.6 r=0.2, 500 x 200 segments = 200,000 triangles, every vertex
+displaced along its normal by N(0, 0.005^2) (SplitMix64 seed 2); query set: the 256^3 grid on
+[-1,1]^3 (16,777,216 FP32 nodes); output: signed distance + closest triangle per node.
+
+A step is one full job on inputs already resident in HBM: LBVH build + FP64 pseudonormals from the
+device mesh arrays, then the 256^3 signed query in FP32 + FP64 fixup, outputs written to HBM.
+`value` = queries / step time.  Also reported: the same step in plain FP32 (no fixup), the query
+alone on a resident BVH, the reference-shaped host call (upload, build, query, download), the
+roofline of the query kernel and the CPU baseline (the oracle's restatement of
+edf_celllist<float>, timed on this host on a seeded subsample of the same grid).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mqueries/s @256^3×200k tris (FP32+fixup vs plain); served p50/p99 latency"
+FLOPS_BOX, FLOPS_TRI, FLOPS_SIGN = 8, 81, 8
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(args):
+    from ssdf_amd import meshgen as mg
+    m = mg.torus(0.6, 0.2, 500, 200, sigma=0.005, seed=2)
+    n = (args.grid,) * 3
+    return m, n, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+
+
+def run_gpu(args, m, n, lo, hi):
+    import ctypes as C
+
+    import ssdf_amd as S
+    L = S.lib()
+    stream = S.Stream()
+    sp = C.c_void_p(stream.ptr)
+    dev = {k: S.DeviceArray.from_host(getattr(m, k)) for k in ("s0", "s1", "s2", "sx", "sy", "sz")}
+    N = n[0] * n[1] * n[2]
+    out = S.DeviceArray(N, np.float32)
+    tri = S.DeviceArray(N, np.int32)
+    ev = [S.Event() for _ in range(4)]
+
+    def create():
+        h = C.c_void_p()
+        rc = L.ssdf_mesh_create(0, m.ntris, C.c_void_p(dev["s0"].ptr), C.c_void_p(dev["s1"].ptr),
+                                C.c_void_p(dev["s2"].ptr), m.nverts, C.c_void_p(dev["sx"].ptr),
+                                C.c_void_p(dev["sy"].ptr), C.c_void_p(dev["sz"].ptr), 1, 1, sp, C.byref(h))
+        S._check(rc, "ssdf_mesh_create")
+        return h
+
+    def query(h, flags):
+        rc = L.ssdf_mesh_query_grid(h, S.MODE_SDF, n[0], n[1], n[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
+                                    C.c_void_p(out.ptr), C.c_void_p(tri.ptr), flags, sp)
+        S._check(rc, "ssdf_mesh_query_grid")
+
+    def step(flags, qtimes):
+        h = create()
+        ev[0].record(stream)
+        query(h, flags)
+        ev[1].record(stream)
+        stream.synchronize()
+        qtimes.append(ev[0].elapsed_ms(ev[1]))
+        L.ssdf_mesh_destroy(h)
+
+    res = {}
+    for name, flags in (("fixup", S.FLAG_FIXUP), ("plain", 0)):
+        qt = []
+        for _ in range(args.warmup):
+            step(flags, qt)
+        qt = []
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(flags, qt)
+        stream.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": dt * 1e3, "query_ms": float(np.mean(qt)), "query_ms_min": float(np.min(qt)),
+                     "mqps": N / dt / 1e6, "query_mqps": N / (np.mean(qt) * 1e-3) / 1e6}
+        log(f"[bench] {name}: step {dt * 1e3:.3f} ms, query {np.mean(qt):.3f} ms")
+    # fixup statistics of one more fixup query on a resident mesh
+    h = create()
+    query(h, S.FLAG_FIXUP)
+    stream.synchronize()
+    q, e = C.c_uint(), C.c_int()
+    L.ssdf_mesh_last_stats(h, C.byref(q), C.byref(e))
+    info = S._MeshInfo()
+    L.ssdf_mesh_info(h, C.byref(info))
+    res["fixup_queued"] = int(q.value)
+    res["kernel_error"] = int(e.value)
+    res["maxdepth"] = int(info.maxdepth)
+    res["open_edges"] = int(info.open_edges)
+    L.ssdf_mesh_destroy(h)
+    # the reference-shaped host call: upload, build, query, download per call (serial/bvh/bvh.cu:80-158)
+    outh = np.empty(N, np.float32)
+    t0 = time.perf_counter()
+    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, n[0], n[1], n[2], *lo, *hi, outh, None, flags=0)
+    res["host_call_plain_s"] = time.perf_counter() - t0
+    res["host_call_plain_mqps"] = N / res["host_call_plain_s"] / 1e6
+    return res
+
+
+def cpu_baseline(args, m, n, lo, hi):
+    """The oracle's edf_celllist<float> restatement (serial/ssdf.hpp:642-963) on a seeded uniform
+    subsample of the grid, all host threads."""
+    from oracle import oracle as O
+    from ssdf_amd import meshgen as mg
+    rng = mg.SplitMix64(12345)
+    N = n[0] * n[1] * n[2]
+    k = args.cpu_sample
+    idx = np.unique((rng.uniform(k) * N).astype(np.int64))
+    ax = [mg.grid_axis(n[d], lo[d], hi[d]) for d in range(3)]
+    i, j, kk = idx % n[0], (idx // n[0]) % n[1], idx // (n[0] * n[1])
+    x, y, z = ax[0][i], ax[1][j], ax[2][kk]
+    out = np.full(idx.size, np.finfo(np.float32).max, np.float32)
+    t0 = time.perf_counter()
+    O.celllist(x, y, z, m, out)
+    dt = time.perf_counter() - t0
+    return {"value": idx.size / dt / 1e6, "unit": "Mqueries/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{idx.size} grid nodes drawn uniformly (SplitMix64 seed 12345) from the 256^3 grid; "
+                      f"edf_celllist<float> restated in C, OpenMP, {dt:.2f} s incl. its cell build"}
+
+
+def roofline(query_ms, n_queries):
+    wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["c2_torus_200k_grid256"]
+    assert wc["queries"] == n_queries
+    flops = FLOPS_BOX * wc["box_tests"] + FLOPS_TRI * wc["tri_tests"] + FLOPS_SIGN * n_queries
+    achieved = flops / (query_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_query_packet (FP32+fixup)",
+            "algorithmic_gflop_per_launch": round(flops / 1e9, 2),
+            "hbm_algorithmic_bytes_per_launch": 8 * n_queries}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (queries shard by "
+              f"replicating the mesh, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    m, n, lo, hi = make_workload(args)
+    N = n[0] * n[1] * n[2]
+    log(f"[bench] mesh {m.name}: {m.ntris} triangles; grid {n}; steps {args.steps} warmup {args.warmup}")
+    g = run_gpu(args, m, n, lo, hi)
+    line = {
+        "metric": METRIC, "value": round(g["fixup"]["mqps"], 2), "unit": "Mqueries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(g["fixup"]["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"noisy torus 200k tris x {n[0]}^3 grid, signed distance + closest triangle, "
+                               "FP32 + FP64 fixup; step = LBVH build + pseudonormals + query",
+                   "triangles": m.ntris, "queries": N, "mesh_checksum": m.checksum()},
+        "plain": {"value": round(g["plain"]["mqps"], 2), "ms_per_step": round(g["plain"]["ms_per_step"], 4),
+                  "query_ms": round(g["plain"]["query_ms"], 4)},
+        "fixup_query_only": {"value": round(g["fixup"]["query_mqps"], 2), "query_ms": round(g["fixup"]["query_ms"], 4)},
+        "host_call_plain": {"value": round(g["host_call_plain_mqps"], 2), "seconds": round(g["host_call_plain_s"], 4)},
+        "fixup_queued": g["fixup_queued"], "kernel_error": g["kernel_error"], "lbvh_maxdepth": g["maxdepth"],
+        "served": None,
+    }
+    if args.grid == 256:
+        line["roofline"] = roofline(g["fixup"]["query_ms"], N)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

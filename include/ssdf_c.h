@@ -1,0 +1,161 @@
+/*
+ * ssdf_c.h — C ABI of the MI355X distance-field engine (libssdf_amd.so).
+ *
+ * Drop-in for zulianp/ssdf's exported symbols plus the entry points its closest-point path needs
+ * on a GPU.  Every function returns 0 on success and a non-zero SSDF_ERR_* code otherwise (the
+ * reference returns int 0 on its hot path and never fails; this engine reports device errors
+ * instead of aborting).  ssdf_last_error() gives the message of the calling thread's last error.
+ *
+ * Layout conventions are the reference's (serial/ssdf.hpp:427-446): SoA coordinates, 0-based int
+ * triangle indices s0, s1, s2, ptrdiff_t counts, x-fastest grids (serial/bvh/bvh.cpp:1792).
+ */
+#ifndef SSDF_C_H
+#define SSDF_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SSDF_OK = 0,
+    SSDF_ERR_INVALID = 1,
+    SSDF_ERR_HIP = 2,
+    SSDF_ERR_OOM = 3,
+    SSDF_ERR_DEPTH = 4,
+    SSDF_ERR_UNSUPPORTED = 5,
+    SSDF_ERR_BUSY = 6
+};
+
+/* query flags */
+#define SSDF_FLAG_FIXUP 1u     /* FP32 traversal + FP64 re-evaluation of near-tied candidates */
+#define SSDF_FLAG_SIGN_FP64 2u /* evaluate every sign in FP64 */
+
+/* query modes of the resident-mesh API */
+#define SSDF_MODE_EDF 0     /* unsigned distance, min-accumulated into out_d */
+#define SSDF_MODE_SDF 1     /* signed distance (pseudonormal sign) + closest triangle, overwriting */
+#define SSDF_MODE_CLOSEST 2 /* distance + closest point + triangle, on strict improvement only */
+
+/* ---------------------------------------------------------------------------------------------
+ * Reference symbols (host pointers, synchronous; results in host memory on return).
+ * --------------------------------------------------------------------------------------------- */
+
+/* Unsigned point-to-surface distance, FP32; out is in/out (min-accumulated), an empty surface
+ * leaves it untouched.  Replaces ssdf_edf_f, serial/ssdf.cpp:36-50 (-> edf_select<float,float,int>,
+ * serial/ssdf.hpp:965-1016).  One call: upload, LBVH build, wave-coherent FP32 query, download. */
+int ssdf_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
+               const float *sz, float *out);
+
+/* FP64 twin; exact FP64 argmin (the reference's CPU edf<double> semantics).  Replaces ssdf_edf_d,
+ * serial/ssdf.cpp:52-66. */
+int ssdf_edf_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const double *sx, const double *sy,
+               const double *sz, double *out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Entry points behind the reference's C++ templates (no C symbol in the reference; the C++
+ * header ssdf/ssdf.hpp wraps these with the reference's template signatures).
+ * --------------------------------------------------------------------------------------------- */
+
+/* closest_point_bvh (serial/bvh/bvh.hpp:34-51, bvh.cpp:125-197): updates outd, outx/y/z and outtri
+ * only where the new distance is strictly below outd. */
+int ssdf_closest_point_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx,
+                         const float *sy, const float *sz, float *outd, float *outx, float *outy, float *outz,
+                         int *outtri, unsigned flags);
+int ssdf_closest_point_d(ptrdiff_t npoints, const double *x, const double *y, const double *z,
+                         ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const double *sx, const double *sy, const double *sz, double *outd, double *outx,
+                         double *outy, double *outz, int *outtri);
+
+/* sample_sdf_bvh (serial/bvh/bvh.hpp:203-227, bvh.cpp:1641-1798): signed distance on the nodes of
+ * an nx x ny x nz grid over [min, max], x fastest, positive where the right-handed normals point.
+ * out_tri (may be NULL) receives the closest triangle.  Overwrites. */
+int ssdf_sample_sdf_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                      const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                      float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, float *out,
+                      int *out_tri, unsigned flags);
+int ssdf_sample_sdf_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                      const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                      ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                      double *out, int *out_tri);
+
+/* Signed distance at arbitrary points (the per-point form of sample_sdf_bvh). */
+int ssdf_sdf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
+               const float *sz, float *out_signed, int *out_tri, unsigned flags);
+
+/* ---------------------------------------------------------------------------------------------
+ * Resident meshes (no reference counterpart: the reference rebuilds its BVH on every call,
+ * serial/bvh/bvh.cu:80-158).  A mesh handle keeps its LBVH (and pseudonormals) in HBM; queries take
+ * DEVICE pointers and run asynchronously on `stream` (a hipStream_t, NULL = default stream).
+ * Calls on one handle must not overlap (they share its fixup workspace).
+ * --------------------------------------------------------------------------------------------- */
+typedef struct ssdf_mesh *ssdf_mesh_t;
+
+typedef struct {
+    ptrdiff_t nselements, nspoints;
+    int dtype;           /* 0 float, 1 double */
+    int maxdepth;        /* LBVH depth */
+    int open_edges;      /* edges not shared by exactly two faces (-1: normals not built) */
+    int flipped_edges;   /* shared edges traversed the same way by both faces */
+    float bbox[6];       /* min xyz, max xyz */
+    size_t device_bytes; /* HBM held by the handle */
+} ssdf_mesh_info_t;
+
+/* dtype 0: float vertices, 1: double.  device_ptrs != 0: the arrays are device memory and are
+ * copied on the device.  with_normals != 0: also build the FP64 pseudonormals (needed by SDF). */
+int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *s1, const int *s2,
+                     ptrdiff_t nspoints, const void *sx, const void *sy, const void *sz, int device_ptrs,
+                     int with_normals, void *stream, ssdf_mesh_t *out);
+int ssdf_mesh_destroy(ssdf_mesh_t mesh);
+int ssdf_mesh_info(ssdf_mesh_t mesh, ssdf_mesh_info_t *info);
+
+/* Points (device SoA, mesh precision).  out_tri, out_cx/cy/cz may be NULL where the mode does not
+ * write them (EDF, SDF).  EDF/CLOSEST read out_d as the current best. */
+int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const void *x, const void *y,
+                           const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy, void *out_cz,
+                           unsigned flags, void *stream);
+
+/* Grid nodes x_i = min + i (max - min) / (n - 1), computed in the mesh precision. */
+int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,
+                         double ymin, double zmin, double xmax, double ymax, double zmax, void *out_d, int *out_tri,
+                         unsigned flags, void *stream);
+
+/* Statistics of the last query on this handle (after its stream has completed): number of
+ * queries re-traversed in FP64 because their candidate list overflowed, and the kernel error flag
+ * (non-zero if a traversal stack overflowed). */
+int ssdf_mesh_last_stats(ssdf_mesh_t mesh, unsigned *fixup_queued, int *kernel_error);
+
+/* ---------------------------------------------------------------------------------------------
+ * Device plumbing for callers without a GPU runtime of their own (tests, benchmarks, FFI).
+ * --------------------------------------------------------------------------------------------- */
+int ssdf_device_count(int *count);
+int ssdf_device_malloc(void **ptr, size_t bytes);
+int ssdf_device_free(void *ptr);
+int ssdf_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ssdf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ssdf_stream_create(void **stream);
+int ssdf_stream_destroy(void *stream);
+int ssdf_stream_synchronize(void *stream);
+/* Device-side timing on a stream: record start, run work, record stop, read milliseconds. */
+int ssdf_event_create(void **event);
+int ssdf_event_destroy(void *event);
+int ssdf_event_record(void *event, void *stream);
+int ssdf_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* GPU brute force (test reference): exact FP64 argmin over every triangle of a resident FP32
+ * mesh, lowest index on ties; out_d2 and out_tri are device arrays. */
+int ssdf_mesh_brute_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const float *x, const float *y, const float *z,
+                           double *out_d2, int *out_tri, void *stream);
+
+const char *ssdf_last_error(void);
+const char *ssdf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSDF_C_H */

@@ -1,0 +1,354 @@
+"""ssdf_amd — MI355X-native closest-point / signed-distance engine.
+
+Python mirror of zulianp/ssdf's hot-path interface over the C ABI of libssdf_amd.so (ctypes).
+The functions keep the reference's names and argument meaning (serial/ssdf.hpp:427-460,
+serial/bvh/bvh.hpp:19-227): SoA query coordinates, int32 triangle indices s0, s1, s2, SoA vertex
+coordinates, caller-owned outputs, `out` in/out for the unsigned `edf` family.  Floating-point
+precision follows the arrays' dtype (float32 -> the FP32 path, float64 -> the exact FP64 path).
+Errors raise SsdfError (the reference returns an int; the C ABI here returns its code).
+
+There is no CPU fallback: importing works anywhere, but every call needs the built library and a
+GPU and fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "SsdfError", "lib", "available", "edf", "edf_celllist", "edf_select", "edf_bvh", "closest_point_bvh",
+    "sample_sdf_bvh", "sdf", "ResidentMesh", "DeviceArray", "Stream", "Event", "FLAG_FIXUP", "FLAG_SIGN_FP64",
+    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST",
+]
+
+FLAG_FIXUP = 1
+FLAG_SIGN_FP64 = 2
+MODE_EDF, MODE_SDF, MODE_CLOSEST = 0, 1, 2
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libssdf_amd.so"
+_lib = None
+
+_P = C.c_void_p
+_PD = C.c_ssize_t  # ptrdiff_t
+
+
+class SsdfError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str = ""):
+        super().__init__(f"{where} failed with code {code}: {msg}")
+        self.code = code
+
+
+class _MeshInfo(C.Structure):
+    _fields_ = [("nselements", _PD), ("nspoints", _PD), ("dtype", C.c_int), ("maxdepth", C.c_int),
+                ("open_edges", C.c_int), ("flipped_edges", C.c_int), ("bbox", C.c_float * 6),
+                ("device_bytes", C.c_size_t)]
+
+
+_SIGS = {
+    "ssdf_edf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
+    "ssdf_edf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
+    "ssdf_closest_point_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint],
+    "ssdf_closest_point_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P, _P],
+    "ssdf_sample_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD, C.c_float, C.c_float, C.c_float,
+                          C.c_float, C.c_float, C.c_float, _P, _P, C.c_uint],
+    "ssdf_sample_sdf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_double, C.c_double, _P, _P],
+    "ssdf_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, C.c_uint],
+    "ssdf_mesh_create": [C.c_int, _PD, _P, _P, _P, _PD, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
+    "ssdf_mesh_destroy": [_P],
+    "ssdf_mesh_info": [_P, C.POINTER(_MeshInfo)],
+    "ssdf_mesh_query_points": [_P, C.c_int, _PD, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
+    "ssdf_mesh_query_grid": [_P, C.c_int, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double, C.c_double,
+                             C.c_double, C.c_double, _P, _P, C.c_uint, _P],
+    "ssdf_mesh_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
+    "ssdf_mesh_brute_points": [_P, _PD, _P, _P, _P, _P, _P, _P],
+    "ssdf_device_count": [C.POINTER(C.c_int)],
+    "ssdf_device_malloc": [C.POINTER(_P), C.c_size_t],
+    "ssdf_device_free": [_P],
+    "ssdf_memcpy_h2d": [_P, _P, C.c_size_t],
+    "ssdf_memcpy_d2h": [_P, _P, C.c_size_t],
+    "ssdf_stream_create": [C.POINTER(_P)],
+    "ssdf_stream_destroy": [_P],
+    "ssdf_stream_synchronize": [_P],
+    "ssdf_event_create": [C.POINTER(_P)],
+    "ssdf_event_destroy": [_P],
+    "ssdf_event_record": [_P, _P],
+    "ssdf_event_elapsed_ms": [_P, _P, C.POINTER(C.c_float)],
+    "ssdf_last_error": [],
+    "ssdf_version": [],
+}
+
+
+def lib() -> C.CDLL:
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        path = Path(os.environ.get("SSDF_AMD_LIB", _LIB_PATH))
+        if not path.exists():
+            raise ImportError(f"libssdf_amd.so not built at {path}; run __graft_entry__.build()")
+        L = C.CDLL(str(path))
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name in ("ssdf_last_error", "ssdf_version") else C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        msg = lib().ssdf_last_error()
+        raise SsdfError(rc, where, msg.decode() if msg else "")
+
+
+def available() -> bool:
+    """True when the library is built and a GPU is visible."""
+    try:
+        n = C.c_int(0)
+        lib().ssdf_device_count(C.byref(n))
+        return n.value > 0
+    except (ImportError, OSError):
+        return False
+
+
+def _ptr(a) -> C.c_void_p | None:
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return C.c_void_p(a.ptr)
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------------------------------------
+# device plumbing
+# ------------------------------------------------------------------------------------------------
+class DeviceArray:
+    """A flat device buffer with a numpy dtype (host <-> device through the C ABI)."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        _check(lib().ssdf_device_malloc(C.byref(p), max(1, self.n * self.dtype.itemsize)), "ssdf_device_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        if self.n:
+            _check(lib().ssdf_memcpy_h2d(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), a.nbytes), "h2d")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        if self.n:
+            _check(lib().ssdf_memcpy_d2h(C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ssdf_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        _check(lib().ssdf_stream_create(C.byref(p)), "ssdf_stream_create")
+        self.ptr = p.value
+
+    def synchronize(self):
+        _check(lib().ssdf_stream_synchronize(C.c_void_p(self.ptr)), "ssdf_stream_synchronize")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().ssdf_stream_destroy(C.c_void_p(self.ptr))
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        _check(lib().ssdf_event_create(C.byref(p)), "ssdf_event_create")
+        self.ptr = p.value
+
+    def record(self, stream: Stream | None = None):
+        _check(lib().ssdf_event_record(C.c_void_p(self.ptr), C.c_void_p(stream.ptr) if stream else None),
+               "ssdf_event_record")
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        _check(lib().ssdf_event_elapsed_ms(C.c_void_p(self.ptr), C.c_void_p(later.ptr), C.byref(ms)), "elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().ssdf_event_destroy(C.c_void_p(self.ptr))
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------------------------
+# reference-shaped entry points (host arrays, synchronous)
+# ------------------------------------------------------------------------------------------------
+def _prec(*arrs):
+    dt = {np.dtype(a.dtype) for a in arrs if a is not None}
+    if dt == {np.dtype(np.float32)}:
+        return "f"
+    if dt == {np.dtype(np.float64)}:
+        return "d"
+    raise TypeError(f"coordinates and outputs must all be float32 or all float64, got {dt}")
+
+
+def _idx(*arrs):
+    for a in arrs:
+        if a.dtype != np.int32:
+            raise TypeError("triangle indices must be int32")
+
+
+def edf(x, y, z, s0, s1, s2, sx, sy, sz, out) -> int:
+    """ssdf::edf (serial/ssdf.hpp:447-460): unsigned distance, min-accumulated into `out`."""
+    p = _prec(x, y, z, sx, sy, sz, out)
+    _idx(s0, s1, s2)
+    fn = lib().ssdf_edf_f if p == "f" else lib().ssdf_edf_d
+    rc = fn(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy),
+            _ptr(sz), _ptr(out))
+    _check(rc, f"ssdf_edf_{p}")
+    return 0
+
+
+# The reference's other engines share edf's contract; on this engine they are the same call.
+edf_celllist = edf
+edf_select = edf
+edf_bvh = edf
+
+
+def closest_point_bvh(x, y, z, s0, s1, s2, sx, sy, sz, outd, outx, outy, outz, outtri, flags: int = 0) -> int:
+    """ssdf::closest_point_bvh (serial/bvh/bvh.hpp:34-51): strict-improvement update of outd, the
+    closest point and the closest triangle."""
+    p = _prec(x, y, z, sx, sy, sz, outd, outx, outy, outz)
+    _idx(s0, s1, s2, outtri)
+    if p == "f":
+        rc = lib().ssdf_closest_point_f(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2),
+                                        sx.size, _ptr(sx), _ptr(sy), _ptr(sz), _ptr(outd), _ptr(outx), _ptr(outy),
+                                        _ptr(outz), _ptr(outtri), flags)
+    else:
+        rc = lib().ssdf_closest_point_d(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2),
+                                        sx.size, _ptr(sx), _ptr(sy), _ptr(sz), _ptr(outd), _ptr(outx), _ptr(outy),
+                                        _ptr(outz), _ptr(outtri))
+    _check(rc, f"ssdf_closest_point_{p}")
+    return 0
+
+
+def sample_sdf_bvh(s0, s1, s2, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, out, out_tri=None,
+                   flags: int = 0) -> int:
+    """ssdf::sample_sdf_bvh (serial/bvh/bvh.hpp:209-227): signed distance on grid nodes, x fastest."""
+    p = _prec(sx, sy, sz, out)
+    _idx(s0, s1, s2)
+    if out.size != nx * ny * nz:
+        raise ValueError("out must hold nx * ny * nz values")
+    if p == "f":
+        rc = lib().ssdf_sample_sdf_f(s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy), _ptr(sz),
+                                     nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, _ptr(out), _ptr(out_tri), flags)
+    else:
+        rc = lib().ssdf_sample_sdf_d(s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy), _ptr(sz),
+                                     nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, _ptr(out), _ptr(out_tri))
+    _check(rc, f"ssdf_sample_sdf_{p}")
+    return 0
+
+
+def sdf(x, y, z, s0, s1, s2, sx, sy, sz, out_signed, out_tri=None, flags: int = 0) -> int:
+    """Signed distance at points (per-point form of sample_sdf_bvh), FP32."""
+    _prec(x, y, z, sx, sy, sz, out_signed)
+    _idx(s0, s1, s2)
+    rc = lib().ssdf_sdf_f(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size,
+                          _ptr(sx), _ptr(sy), _ptr(sz), _ptr(out_signed), _ptr(out_tri), flags)
+    _check(rc, "ssdf_sdf_f")
+    return 0
+
+
+# ------------------------------------------------------------------------------------------------
+# resident meshes (device API)
+# ------------------------------------------------------------------------------------------------
+class ResidentMesh:
+    """A mesh whose LBVH (and pseudonormals) live in HBM; queries take DeviceArrays."""
+
+    def __init__(self, s0, s1, s2, sx, sy, sz, with_normals: bool = True, stream: Stream | None = None):
+        dev = isinstance(sx, DeviceArray)
+        dtype = (sx.dtype if dev else np.asarray(sx).dtype)
+        self.dtype = np.dtype(dtype)
+        self.dt_code = 0 if self.dtype == np.float32 else 1
+        h = C.c_void_p()
+        n = s0.n if dev else s0.size
+        nv = sx.n if dev else sx.size
+        rc = lib().ssdf_mesh_create(self.dt_code, n, _ptr(s0), _ptr(s1), _ptr(s2), nv, _ptr(sx), _ptr(sy),
+                                    _ptr(sz), 1 if dev else 0, 1 if with_normals else 0,
+                                    C.c_void_p(stream.ptr) if stream else None, C.byref(h))
+        _check(rc, "ssdf_mesh_create")
+        self.h = h.value
+
+    @classmethod
+    def from_mesh(cls, m, with_normals=True, dtype=np.float32):
+        return cls(m.s0, m.s1, m.s2, m.sx.astype(dtype), m.sy.astype(dtype), m.sz.astype(dtype), with_normals)
+
+    def info(self) -> dict:
+        i = _MeshInfo()
+        _check(lib().ssdf_mesh_info(C.c_void_p(self.h), C.byref(i)), "ssdf_mesh_info")
+        return dict(nselements=i.nselements, nspoints=i.nspoints, dtype=i.dtype, maxdepth=i.maxdepth,
+                    open_edges=i.open_edges, flipped_edges=i.flipped_edges, bbox=list(i.bbox),
+                    device_bytes=i.device_bytes)
+
+    def query_points(self, mode, x, y, z, out_d, out_tri=None, cx=None, cy=None, cz=None, flags=0,
+                     stream: Stream | None = None):
+        rc = lib().ssdf_mesh_query_points(C.c_void_p(self.h), mode, x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_d),
+                                          _ptr(out_tri), _ptr(cx), _ptr(cy), _ptr(cz), flags,
+                                          C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_query_points")
+
+    def query_grid(self, mode, n, lo, hi, out_d, out_tri=None, flags=0, stream: Stream | None = None):
+        nx, ny, nz = n
+        rc = lib().ssdf_mesh_query_grid(C.c_void_p(self.h), mode, nx, ny, nz, float(lo[0]), float(lo[1]),
+                                        float(lo[2]), float(hi[0]), float(hi[1]), float(hi[2]), _ptr(out_d),
+                                        _ptr(out_tri), flags, C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_query_grid")
+
+    def brute_points(self, x, y, z, out_d2, out_tri, stream: Stream | None = None):
+        rc = lib().ssdf_mesh_brute_points(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_d2),
+                                          _ptr(out_tri), C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_brute_points")
+
+    def last_stats(self):
+        q, e = C.c_uint(), C.c_int()
+        _check(lib().ssdf_mesh_last_stats(C.c_void_p(self.h), C.byref(q), C.byref(e)), "ssdf_mesh_last_stats")
+        return int(q.value), int(e.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ssdf_mesh_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

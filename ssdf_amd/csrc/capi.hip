@@ -1,0 +1,520 @@
+// capi.hip — the C ABI (include/ssdf_c.h): reference symbols, resident meshes, device plumbing.
+#include "query.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+using namespace ssdf_amd;
+
+namespace {
+
+// FP32 error bound of a computed distance, relative to the largest coordinate magnitude involved:
+// 32 ulps of 1.0 (a few roundings of the projection and of p - q, with ample margin).
+constexpr float kBandRel = 32.0f * 1.1920928955078125e-07f;
+constexpr float kTau = 1e-5f;           // relative near-tie band on distances
+constexpr double kDelta64Rel = 1e-13;   // FP64 pruning margin relative to the coordinate scale
+constexpr double kSignEpsRel = 1e-6;    // FP64 sign within this fraction of the bbox diagonal
+
+struct Workspace {
+    unsigned *status = nullptr;  // [0] fixup count, [1] kernel error flag
+    unsigned *list = nullptr;
+    size_t cap = 0;
+};
+
+__global__ void k_reset_status(unsigned *status) {
+    if (threadIdx.x < 2) status[threadIdx.x] = 0u;
+}
+
+int ws_reserve(Workspace &w, size_t n) {
+    if (!w.status) SSDF_HIP_TRY(hipMalloc(&w.status, 64));
+    if (w.cap < n) {
+        if (w.list) SSDF_HIP_TRY(hipFree(w.list));
+        w.list = nullptr;
+        w.cap = 0;
+        SSDF_HIP_TRY(hipMalloc(&w.list, sizeof(unsigned) * std::max<size_t>(n, 1)));
+        w.cap = n;
+    }
+    return SSDF_OK;
+}
+
+void ws_free(Workspace &w) {
+    (void)hipFree(w.status);
+    (void)hipFree(w.list);
+    w = Workspace{};
+}
+
+// Exact FP64 argmin over every triangle (tiles of 256 TriF in LDS).
+__global__ __launch_bounds__(256) void k_brute_f64(const TriF *__restrict__ tris, int ntri,
+                                                   const float *__restrict__ x, const float *__restrict__ y,
+                                                   const float *__restrict__ z, long long npoints,
+                                                   double *__restrict__ out_d2, int *__restrict__ out_tri) {
+    __shared__ TriF tile[256];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = p < npoints;
+    const double px = ok ? (double)x[p] : 0.0, py = ok ? (double)y[p] : 0.0, pz = ok ? (double)z[p] : 0.0;
+    double best = __builtin_inf();
+    int bt = 0x7fffffff;
+    for (int base = 0; base < ntri; base += 256) {
+        __syncthreads();
+        if (base + (int)threadIdx.x < ntri) tile[threadIdx.x] = tris[base + threadIdx.x];
+        __syncthreads();
+        const int m = std::min(256, ntri - base);
+        for (int i = 0; i < m; i++) {
+            const TriF &tr = tile[i];
+            const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2],
+                                               tr.c[0], tr.c[1], tr.c[2]);
+            if (d < best || (d == best && tr.orig < bt)) best = d, bt = tr.orig;
+        }
+    }
+    if (ok) {
+        out_d2[p] = best;
+        out_tri[p] = bt == 0x7fffffff ? -1 : bt;
+    }
+}
+
+template <typename G>
+int upload_mesh(DevMesh &m, ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp,
+                const G *sx, const G *sy, const G *sz, bool device_ptrs, hipStream_t stream) {
+    m.dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    m.nsel = nsel;
+    m.nsp = nsp;
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const size_t ib = sizeof(int) * (size_t)nsel, vb = sizeof(G) * (size_t)nsp;
+    int **idx[3] = {&m.s0, &m.s1, &m.s2};
+    const int *isrc[3] = {s0, s1, s2};
+    for (int k = 0; k < 3; k++) {
+        SSDF_HIP_TRY(hipMalloc(idx[k], std::max<size_t>(ib, 4)));
+        SSDF_HIP_TRY(hipMemcpyAsync(*idx[k], isrc[k], ib, kind, stream));
+    }
+    void **vtx[3] = {&m.sx, &m.sy, &m.sz};
+    const G *vsrc[3] = {sx, sy, sz};
+    for (int k = 0; k < 3; k++) {
+        SSDF_HIP_TRY(hipMalloc(vtx[k], std::max<size_t>(vb, 8)));
+        SSDF_HIP_TRY(hipMemcpyAsync(*vtx[k], vsrc[k], vb, kind, stream));
+    }
+    m.owns_input = true;
+    m.device_bytes += 3 * (ib + vb);
+    return SSDF_OK;
+}
+
+float mesh_scale(const DevMesh &m) {
+    float L = 1.0f;
+    for (int d = 0; d < 6; d++) L = std::max(L, std::fabs(ordered_to_float(m.hmeta.bbox_ord[d])));
+    return L;
+}
+
+float mesh_diag(const DevMesh &m) {
+    double s = 0;
+    for (int d = 0; d < 3; d++) {
+        const double e = (double)ordered_to_float(m.hmeta.bbox_ord[3 + d]) - ordered_to_float(m.hmeta.bbox_ord[d]);
+        s += e * e;
+    }
+    return (float)std::sqrt(s);
+}
+
+}  // namespace
+
+struct ssdf_mesh {
+    DevMesh m;
+    Workspace ws;
+};
+
+namespace {
+
+int prepare_args(ssdf_mesh *h, QueryArgs &a, size_t nq, unsigned flags, hipStream_t stream) {
+    if (int rc = ws_reserve(h->ws, (flags & QF_FIXUP) ? nq : 1)) return rc;
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, h->ws.status);
+    a.fix_count = h->ws.status;
+    a.err_flag = (int *)(h->ws.status + 1);
+    a.fix_list = h->ws.list;
+    a.fix_cap = (unsigned)std::min<size_t>(h->ws.cap, 0xffffffffu);
+    const float L = mesh_scale(h->m);
+    a.tau = kTau;
+    a.band_abs = kBandRel * L;
+    a.sign_eps = (float)(kSignEpsRel * mesh_diag(h->m));
+    a.delta64 = kDelta64Rel * L;
+    return SSDF_OK;
+}
+
+int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp,
+                const void *sx, const void *sy, const void *sz, bool device_ptrs, bool with_normals,
+                hipStream_t stream, ssdf_mesh **out) {
+    *out = nullptr;
+    if (nsel < 0 || nsp < 0 || (nsel > 0 && (!s0 || !s1 || !s2 || !sx || !sy || !sz))) {
+        set_error("ssdf_mesh_create: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    ssdf_mesh *h = new (std::nothrow) ssdf_mesh();
+    if (!h) return SSDF_ERR_OOM;
+    int rc = dtype == 0 ? upload_mesh<float>(h->m, nsel, s0, s1, s2, nsp, (const float *)sx, (const float *)sy,
+                                             (const float *)sz, device_ptrs, stream)
+                        : upload_mesh<double>(h->m, nsel, s0, s1, s2, nsp, (const double *)sx, (const double *)sy,
+                                              (const double *)sz, device_ptrs, stream);
+    if (!rc && nsel > 0) rc = build_lbvh(h->m, stream);
+    if (!rc && nsel > 0 && with_normals) rc = build_normals(h->m, stream);
+    if (!rc && nsel > 0) rc = sync_meta(h->m, stream);
+    if (rc) {
+        free_mesh(h->m);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SSDF_OK;
+}
+
+void destroy_mesh(ssdf_mesh *h) {
+    if (!h) return;
+    free_mesh(h->m);
+    ws_free(h->ws);
+    delete h;
+}
+
+int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y, const void *z, void *out_d,
+                 int *out_tri, void *cx, void *cy, void *cz, unsigned flags, hipStream_t stream) {
+    if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
+    if (!x || !y || !z || !out_d || (mode == SSDF_MODE_CLOSEST && (!cx || !cy || !cz))) {
+        set_error("query_points: null argument");
+        return SSDF_ERR_INVALID;
+    }
+    QueryArgs a{};
+    if (int rc = prepare_args(h, a, (size_t)n, flags, stream)) return rc;
+    a.qx = x, a.qy = y, a.qz = z;
+    a.npoints = n;
+    a.out_d = out_d;
+    a.out_tri = out_tri;
+    a.out_cx = cx, a.out_cy = cy, a.out_cz = cz;
+    const int dbl = h->m.dtype;
+    return launch_query(h->m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream);
+}
+
+int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, const double *lo, const double *hi,
+               void *out_d, int *out_tri, unsigned flags, hipStream_t stream) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || h->m.nsel <= 0) return SSDF_OK;
+    if (nx > 0x7fffffff || ny > 0x7fffffff || nz > 0x7fffffff || !out_d) {
+        set_error("query_grid: invalid grid");
+        return SSDF_ERR_INVALID;
+    }
+    QueryArgs a{};
+    const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
+    if (int rc = prepare_args(h, a, nq, flags, stream)) return rc;
+    a.nx = (int)nx, a.ny = (int)ny, a.nz = (int)nz;
+    const ptrdiff_t ns[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; d++) {
+        // spacing in the grid's precision (serial/bvh/bvh.cpp:1665-1667)
+        if (h->m.dtype == 0) {
+            const float l = (float)lo[d], u = (float)hi[d];
+            a.gmin[d] = l;
+            a.gh[d] = ns[d] > 1 ? (double)((u - l) / (float)(ns[d] - 1)) : 0.0;
+        } else {
+            a.gmin[d] = lo[d];
+            a.gh[d] = ns[d] > 1 ? (hi[d] - lo[d]) / (double)(ns[d] - 1) : 0.0;
+        }
+    }
+    a.out_d = out_d;
+    a.out_tri = out_tri;
+    const int dbl = h->m.dtype;
+    return launch_query(h->m, a, QSRC_GRID, mode, flags, dbl, dbl, stream);
+}
+
+// RAII holder of device buffers for the synchronous host-pointer entry points.
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+template <typename G>
+int h2d(DevBuf &b, const G *src, ptrdiff_t n) {
+    SSDF_HIP_TRY(hipMalloc(&b.p, sizeof(G) * (size_t)std::max<ptrdiff_t>(n, 1)));
+    if (src && n > 0) SSDF_HIP_TRY(hipMemcpy(b.p, src, sizeof(G) * (size_t)n, hipMemcpyHostToDevice));
+    return SSDF_OK;
+}
+
+template <typename G>
+int d2h(G *dst, const DevBuf &b, ptrdiff_t n) {
+    if (dst && n > 0) SSDF_HIP_TRY(hipMemcpy(dst, b.p, sizeof(G) * (size_t)n, hipMemcpyDeviceToHost));
+    return SSDF_OK;
+}
+
+// The reference's per-call GPU structure (serial/bvh/bvh.cu:80-158): allocate, upload, build,
+// query, download, free.  Results are in host memory on return.
+template <typename G>
+int host_points_call(int mode, ptrdiff_t npoints, const G *x, const G *y, const G *z, ptrdiff_t nsel, const int *s0,
+                     const int *s1, const int *s2, ptrdiff_t nsp, const G *sx, const G *sy, const G *sz, G *outd,
+                     G *outx, G *outy, G *outz, int *outtri, unsigned flags) {
+    if (nsel == 0 || nsp == 0 || npoints <= 0) return SSDF_OK;  // serial/ssdf.hpp:462
+    if (!x || !y || !z || !outd) {
+        set_error("null point or output array");
+        return SSDF_ERR_INVALID;
+    }
+    hipStream_t stream = nullptr;
+    ssdf_mesh *h = nullptr;
+    const int dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, mode == SSDF_MODE_SDF, stream, &h))
+        return rc;
+    struct Guard {
+        ssdf_mesh *h;
+        ~Guard() { destroy_mesh(h); }
+    } guard{h};
+    DevBuf dx, dy, dz, dd, dt, dcx, dcy, dcz;
+    if (int rc = h2d(dx, x, npoints)) return rc;
+    if (int rc = h2d(dy, y, npoints)) return rc;
+    if (int rc = h2d(dz, z, npoints)) return rc;
+    if (int rc = h2d(dd, mode == SSDF_MODE_SDF ? (const G *)nullptr : outd, npoints)) return rc;
+    if (outtri) {
+        if (int rc = h2d(dt, (const int *)nullptr, npoints)) return rc;
+        if (mode == SSDF_MODE_CLOSEST) SSDF_HIP_TRY(hipMemcpy(dt.p, outtri, sizeof(int) * npoints, hipMemcpyHostToDevice));
+    }
+    if (mode == SSDF_MODE_CLOSEST) {
+        if (int rc = h2d(dcx, outx, npoints)) return rc;
+        if (int rc = h2d(dcy, outy, npoints)) return rc;
+        if (int rc = h2d(dcz, outz, npoints)) return rc;
+    }
+    if (int rc = query_points(h, mode, npoints, dx.p, dy.p, dz.p, dd.p, (int *)dt.p, dcx.p, dcy.p, dcz.p, flags,
+                              stream))
+        return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    if (int rc = d2h(outd, dd, npoints)) return rc;
+    if (outtri)
+        if (int rc = d2h(outtri, dt, npoints)) return rc;
+    if (mode == SSDF_MODE_CLOSEST) {
+        if (int rc = d2h(outx, dcx, npoints)) return rc;
+        if (int rc = d2h(outy, dcy, npoints)) return rc;
+        if (int rc = d2h(outz, dcz, npoints)) return rc;
+    }
+    return SSDF_OK;
+}
+
+template <typename G>
+int host_grid_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp, const G *sx,
+                   const G *sy, const G *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, G xmin, G ymin, G zmin, G xmax,
+                   G ymax, G zmax, G *out, int *out_tri, unsigned flags) {
+    if (nsel == 0 || nsp == 0 || nx <= 0 || ny <= 0 || nz <= 0) return SSDF_OK;
+    if (!out) {
+        set_error("null output array");
+        return SSDF_ERR_INVALID;
+    }
+    hipStream_t stream = nullptr;
+    ssdf_mesh *h = nullptr;
+    const int dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, true, stream, &h)) return rc;
+    struct Guard {
+        ssdf_mesh *h;
+        ~Guard() { destroy_mesh(h); }
+    } guard{h};
+    const ptrdiff_t nq = nx * ny * nz;
+    DevBuf dd, dt;
+    if (int rc = h2d(dd, (const G *)nullptr, nq)) return rc;
+    if (out_tri)
+        if (int rc = h2d(dt, (const int *)nullptr, nq)) return rc;
+    const double lo[3] = {(double)xmin, (double)ymin, (double)zmin}, hi[3] = {(double)xmax, (double)ymax, (double)zmax};
+    if (int rc = query_grid(h, SSDF_MODE_SDF, nx, ny, nz, lo, hi, dd.p, (int *)dt.p, flags, stream)) return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    if (int rc = d2h(out, dd, nq)) return rc;
+    if (out_tri)
+        if (int rc = d2h(out_tri, dt, nq)) return rc;
+    return SSDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssdf_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
+               const float *sz, float *out) {
+    return host_points_call<float>(SSDF_MODE_EDF, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz,
+                                   out, nullptr, nullptr, nullptr, nullptr, 0u);
+}
+
+int ssdf_edf_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const double *sx, const double *sy,
+               const double *sz, double *out) {
+    return host_points_call<double>(SSDF_MODE_EDF, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz,
+                                    out, nullptr, nullptr, nullptr, nullptr, 0u);
+}
+
+int ssdf_closest_point_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx,
+                         const float *sy, const float *sz, float *outd, float *outx, float *outy, float *outz,
+                         int *outtri, unsigned flags) {
+    if (!outx || !outy || !outz) {
+        set_error("ssdf_closest_point_f: null closest-point output");
+        return SSDF_ERR_INVALID;
+    }
+    return host_points_call<float>(SSDF_MODE_CLOSEST, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy,
+                                   sz, outd, outx, outy, outz, outtri, flags);
+}
+
+int ssdf_closest_point_d(ptrdiff_t npoints, const double *x, const double *y, const double *z,
+                         ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const double *sx, const double *sy, const double *sz, double *outd, double *outx,
+                         double *outy, double *outz, int *outtri) {
+    if (!outx || !outy || !outz) {
+        set_error("ssdf_closest_point_d: null closest-point output");
+        return SSDF_ERR_INVALID;
+    }
+    return host_points_call<double>(SSDF_MODE_CLOSEST, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy,
+                                    sz, outd, outx, outy, outz, outtri, 0u);
+}
+
+int ssdf_sample_sdf_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                      const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                      float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, float *out,
+                      int *out_tri, unsigned flags) {
+    return host_grid_call<float>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                 ymax, zmax, out, out_tri, flags);
+}
+
+int ssdf_sample_sdf_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                      const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                      ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                      double *out, int *out_tri) {
+    return host_grid_call<double>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                  ymax, zmax, out, out_tri, 0u);
+}
+
+int ssdf_sdf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+               const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
+               const float *sz, float *out_signed, int *out_tri, unsigned flags) {
+    return host_points_call<float>(SSDF_MODE_SDF, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz,
+                                   out_signed, nullptr, nullptr, nullptr, out_tri, flags);
+}
+
+int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *s1, const int *s2,
+                     ptrdiff_t nspoints, const void *sx, const void *sy, const void *sz, int device_ptrs,
+                     int with_normals, void *stream, ssdf_mesh_t *out) {
+    if (!out || (dtype != 0 && dtype != 1)) {
+        set_error("ssdf_mesh_create: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    return create_mesh(dtype, nselements, s0, s1, s2, nspoints, sx, sy, sz, device_ptrs != 0, with_normals != 0,
+                       (hipStream_t)stream, out);
+}
+
+int ssdf_mesh_destroy(ssdf_mesh_t mesh) {
+    destroy_mesh(mesh);
+    return SSDF_OK;
+}
+
+int ssdf_mesh_info(ssdf_mesh_t mesh, ssdf_mesh_info_t *info) {
+    if (!mesh || !info) return SSDF_ERR_INVALID;
+    if (mesh->m.nsel > 0)
+        if (int rc = sync_meta(mesh->m, nullptr)) return rc;
+    info->nselements = mesh->m.nsel;
+    info->nspoints = mesh->m.nsp;
+    info->dtype = mesh->m.dtype;
+    info->maxdepth = mesh->m.hmeta.maxdepth;
+    info->open_edges = mesh->m.has_normals ? mesh->m.hmeta.open_edges : -1;
+    info->flipped_edges = mesh->m.has_normals ? mesh->m.hmeta.flipped_edges : -1;
+    for (int d = 0; d < 6; d++) info->bbox[d] = ordered_to_float(mesh->m.hmeta.bbox_ord[d]);
+    info->device_bytes = mesh->m.device_bytes;
+    return SSDF_OK;
+}
+
+int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const void *x, const void *y,
+                           const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy, void *out_cz,
+                           unsigned flags, void *stream) {
+    if (!mesh || mode < 0 || mode > 2) {
+        set_error("ssdf_mesh_query_points: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    return query_points(mesh, mode, npoints, x, y, z, out_d, out_tri, out_cx, out_cy, out_cz, flags,
+                        (hipStream_t)stream);
+}
+
+int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,
+                         double ymin, double zmin, double xmax, double ymax, double zmax, void *out_d, int *out_tri,
+                         unsigned flags, void *stream) {
+    if (!mesh || (mode != SSDF_MODE_SDF && mode != SSDF_MODE_EDF)) {
+        set_error("ssdf_mesh_query_grid: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    const double lo[3] = {xmin, ymin, zmin}, hi[3] = {xmax, ymax, zmax};
+    return query_grid(mesh, mode, nx, ny, nz, lo, hi, out_d, out_tri, flags, (hipStream_t)stream);
+}
+
+int ssdf_mesh_last_stats(ssdf_mesh_t mesh, unsigned *fixup_queued, int *kernel_error) {
+    if (!mesh) return SSDF_ERR_INVALID;
+    unsigned st[2] = {0, 0};
+    if (mesh->ws.status) SSDF_HIP_TRY(hipMemcpy(st, mesh->ws.status, sizeof(st), hipMemcpyDeviceToHost));
+    if (fixup_queued) *fixup_queued = st[0];
+    if (kernel_error) *kernel_error = (int)st[1];
+    return SSDF_OK;
+}
+
+int ssdf_mesh_brute_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const float *x, const float *y, const float *z,
+                           double *out_d2, int *out_tri, void *stream) {
+    if (!mesh || mesh->m.dtype != 0) {
+        set_error("ssdf_mesh_brute_points: needs an FP32 mesh");
+        return SSDF_ERR_INVALID;
+    }
+    if (npoints <= 0) return SSDF_OK;
+    const unsigned nb = (unsigned)((npoints + 255) / 256);
+    hipLaunchKernelGGL(k_brute_f64, dim3(nb), dim3(256), 0, (hipStream_t)stream, mesh->m.tris, (int)mesh->m.nsel,
+                       x, y, z, (long long)npoints, out_d2, out_tri);
+    SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
+}
+
+int ssdf_device_count(int *count) {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    return SSDF_OK;
+}
+int ssdf_device_malloc(void **ptr, size_t bytes) {
+    SSDF_HIP_TRY(hipMalloc(ptr, std::max<size_t>(bytes, 4)));
+    return SSDF_OK;
+}
+int ssdf_device_free(void *ptr) {
+    SSDF_HIP_TRY(hipFree(ptr));
+    return SSDF_OK;
+}
+int ssdf_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    SSDF_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SSDF_OK;
+}
+int ssdf_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    SSDF_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SSDF_OK;
+}
+int ssdf_stream_create(void **stream) {
+    SSDF_HIP_TRY(hipStreamCreateWithFlags((hipStream_t *)stream, hipStreamNonBlocking));
+    return SSDF_OK;
+}
+int ssdf_stream_destroy(void *stream) {
+    SSDF_HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return SSDF_OK;
+}
+int ssdf_stream_synchronize(void *stream) {
+    SSDF_HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return SSDF_OK;
+}
+int ssdf_event_create(void **event) {
+    SSDF_HIP_TRY(hipEventCreate((hipEvent_t *)event));
+    return SSDF_OK;
+}
+int ssdf_event_destroy(void *event) {
+    SSDF_HIP_TRY(hipEventDestroy((hipEvent_t)event));
+    return SSDF_OK;
+}
+int ssdf_event_record(void *event, void *stream) {
+    SSDF_HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return SSDF_OK;
+}
+int ssdf_event_elapsed_ms(void *start, void *stop, float *ms) {
+    SSDF_HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+    SSDF_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SSDF_OK;
+}
+
+const char *ssdf_last_error(void) { return get_error(); }
+const char *ssdf_version(void) { return "ssdf_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,556 @@
+// lbvh.hip — LBVH construction and pseudonormals for a mesh resident in HBM.
+//
+// Build (one stream, no host round trip, graph-capturable):
+//   k_mesh_init   meta to neutral values
+//   k_tri_boxes   per-triangle FP32 AABB (outward-rounded for FP64 vertices) + centre/vertex bounds
+//   k_morton      63-bit Morton key of each box centre, normalised per axis by the centre bounds
+//   radix sort    (key, triangle) pairs, 8 passes of 8 bits, stable -> equal keys keep index order
+//   k_karras      internal node i's children and parent links (Karras, HPG 2012), ties in the keys
+//                 split by position, so the tree is a pure function of the mesh
+//   k_refit_pack  bottom-up: each leaf writes its box into the parent's child slot; the second
+//                 child to arrive at a node (atomic counter) writes the union one level up; the
+//                 leaf pass also packs the Morton-ordered 48-byte TriF records
+//   k_depth       deepest leaf (sizes the traversal stack)
+// The same construction, run serially, is the oracle's canonical replay (oracle/oracle.c), so the
+// device tree and the replay's tree are identical.
+#include "mesh.hpp"
+#include "radix.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace ssdf_amd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *get_error() { return g_last_error.c_str(); }
+
+__global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m) {
+    __shared__ uint32_t s[1024];
+    const unsigned tid = threadIdx.x;
+    const size_t per = (m + 1023) / 1024;
+    const size_t beg = (size_t)tid * per;
+    const size_t end = beg + per < m ? beg + per : m;
+    uint32_t sum = 0;
+    for (size_t i = beg; i < end; i++) sum += data[i];
+    s[tid] = sum;
+    __syncthreads();
+    for (unsigned off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = tid >= off ? s[tid - off] : 0u;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+    for (size_t i = beg; i < end; i++) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ void k_mesh_init(MeshMeta *meta) {
+    if (threadIdx.x == 0) {
+        meta->root = 0;
+        meta->maxdepth = 0;
+        for (int d = 0; d < 3; d++) {
+            meta->bbox_ord[d] = 0x7fffffff;
+            meta->bbox_ord[3 + d] = (int)0x80000000;
+            meta->cbox_ord[d] = 0x7fffffff;
+            meta->cbox_ord[3 + d] = (int)0x80000000;
+            meta->root_box[d] = 0.f;
+            meta->root_box[3 + d] = 0.f;
+        }
+        meta->open_edges = -1;
+        meta->flipped_edges = -1;
+    }
+}
+
+__device__ __forceinline__ float wave_min_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename G>
+__device__ __forceinline__ float to_f_down(G v);
+template <>
+__device__ __forceinline__ float to_f_down<float>(float v) {
+    return v;
+}
+template <>
+__device__ __forceinline__ float to_f_down<double>(double v) {
+    return __double2float_rd(v);
+}
+template <typename G>
+__device__ __forceinline__ float to_f_up(G v);
+template <>
+__device__ __forceinline__ float to_f_up<float>(float v) {
+    return v;
+}
+template <>
+__device__ __forceinline__ float to_f_up<double>(double v) {
+    return __double2float_ru(v);
+}
+
+// Triangle box (6 floats: min xyz, max xyz) + centre bounds and vertex bounds.
+template <typename G>
+__global__ __launch_bounds__(256) void k_tri_boxes(const int *__restrict__ s0, const int *__restrict__ s1,
+                                                   const int *__restrict__ s2, const G *__restrict__ sx,
+                                                   const G *__restrict__ sy, const G *__restrict__ sz, int n,
+                                                   float *__restrict__ boxes, MeshMeta *__restrict__ meta) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float bmn[3] = {SSDF_INF, SSDF_INF, SSDF_INF}, bmx[3] = {-SSDF_INF, -SSDF_INF, -SSDF_INF};
+    float cmn[3] = {SSDF_INF, SSDF_INF, SSDF_INF}, cmx[3] = {-SSDF_INF, -SSDF_INF, -SSDF_INF};
+    if (t < n) {
+        const int iv[3] = {s0[t], s1[t], s2[t]};
+        const G *co[3] = {sx, sy, sz};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const G a = co[d][iv[0]], b = co[d][iv[1]], c = co[d][iv[2]];
+            G lo = b < a ? b : a;
+            lo = c < lo ? c : lo;
+            G hi = a < b ? b : a;
+            hi = hi < c ? c : hi;
+            const float flo = to_f_down<G>(lo), fhi = to_f_up<G>(hi);
+            boxes[(size_t)6 * t + d] = flo;
+            boxes[(size_t)6 * t + 3 + d] = fhi;
+            bmn[d] = flo, bmx[d] = fhi;
+            const float cc = (flo + fhi) * 0.5f;
+            cmn[d] = cc, cmx[d] = cc;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float a = wave_min_f(bmn[d]), b = wave_max_f(bmx[d]);
+        const float c = wave_min_f(cmn[d]), e = wave_max_f(cmx[d]);
+        if ((threadIdx.x & 63) == 0 && a <= b) {
+            atomicMin(&meta->bbox_ord[d], float_to_ordered(a));
+            atomicMax(&meta->bbox_ord[3 + d], float_to_ordered(b));
+            atomicMin(&meta->cbox_ord[d], float_to_ordered(c));
+            atomicMax(&meta->cbox_ord[3 + d], float_to_ordered(e));
+        }
+    }
+}
+
+__host__ __device__ __forceinline__ uint64_t expand21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x001f00000000ffffull;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ boxes, int n,
+                                                const MeshMeta *__restrict__ meta, uint64_t *__restrict__ keys,
+                                                uint32_t *__restrict__ vals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float lo = ordered_to_float(meta->cbox_ord[d]);
+        const float hi = ordered_to_float(meta->cbox_ord[3 + d]);
+        const float ext = hi - lo;
+        const float c = (boxes[(size_t)6 * t + d] + boxes[(size_t)6 * t + 3 + d]) * 0.5f;
+        const float u = ext > 0.0f ? (c - lo) / ext : 0.0f;
+        const float s = u * 2097152.0f;
+        const uint32_t qi = s > 0.0f ? (uint32_t)s : 0u;
+        q[d] = qi > 2097151u ? 2097151u : qi;
+    }
+    keys[t] = (expand21(q[0]) << 2) | (expand21(q[1]) << 1) | expand21(q[2]);
+    vals[t] = (uint32_t)t;
+}
+
+__device__ __forceinline__ int kdelta(const uint64_t *__restrict__ codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = codes[i], b = codes[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz(i ^ j);
+}
+
+// parent[] holds 2n-1 entries: internal node i at i, leaf k at n-1+k; value (p << 1) | side.
+__global__ __launch_bounds__(256) void k_karras(const uint64_t *__restrict__ codes, int n, Node *__restrict__ nodes,
+                                                int *__restrict__ parent, int *__restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    flags[i] = 0;
+    const int dir = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(codes, n, i, i - dir);
+    int lmax = 2;
+    while (kdelta(codes, n, i, i + lmax * dir) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(codes, n, i, i + (l + t) * dir) > dmin) l += t;
+    const int j = i + l * dir;
+    const int dnode = kdelta(codes, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(codes, n, i, i + (s + t) * dir) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * dir + (dir < 0 ? dir : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    int left, right;
+    if (lo == gamma) {
+        left = ~gamma;
+        parent[n - 1 + gamma] = (i << 1) | 0;
+    } else {
+        left = gamma;
+        parent[gamma] = (i << 1) | 0;
+    }
+    if (hi == gamma + 1) {
+        right = ~(gamma + 1);
+        parent[n - 1 + gamma + 1] = (i << 1) | 1;
+    } else {
+        right = gamma + 1;
+        parent[gamma + 1] = (i << 1) | 1;
+    }
+    nodes[i].left = left;
+    nodes[i].right = right;
+    nodes[i].pad0 = 0;
+    nodes[i].pad1 = 0;
+}
+
+template <typename G>
+__device__ __forceinline__ float to_f_near(G v) {
+    return (float)v;
+}
+
+template <typename G>
+__global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__ order, int n,
+                                                    const int *__restrict__ s0, const int *__restrict__ s1,
+                                                    const int *__restrict__ s2, const G *__restrict__ sx,
+                                                    const G *__restrict__ sy, const G *__restrict__ sz,
+                                                    const float *__restrict__ boxes, const int *__restrict__ parent,
+                                                    int *__restrict__ flags, Node *__restrict__ nodes,
+                                                    TriF *__restrict__ tris, MeshMeta *__restrict__ meta) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int t = (int)order[k];
+    {
+        const int iv[3] = {s0[t], s1[t], s2[t]};
+        TriF tr;
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            float *dst = v == 0 ? tr.a : (v == 1 ? tr.b : tr.c);
+            dst[0] = to_f_near<G>(sx[iv[v]]);
+            dst[1] = to_f_near<G>(sy[iv[v]]);
+            dst[2] = to_f_near<G>(sz[iv[v]]);
+        }
+        tr.orig = t;
+        tr.pad0 = 0;
+        tr.pad1 = 0;
+        tris[k] = tr;
+    }
+    float mn[3], mx[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) mn[d] = boxes[(size_t)6 * t + d], mx[d] = boxes[(size_t)6 * t + 3 + d];
+    if (n == 1) {
+        for (int d = 0; d < 3; d++) meta->root_box[d] = mn[d], meta->root_box[3 + d] = mx[d];
+        meta->root = ~0;
+        return;
+    }
+    int p = parent[n - 1 + k];
+    for (int guard = 0; guard < 4096; guard++) {
+        const int pi = p >> 1, side = p & 1;
+        Node *nd = nodes + pi;
+        float *dmn = side ? nd->rmin : nd->lmin;
+        float *dmx = side ? nd->rmax : nd->lmax;
+#pragma unroll
+        for (int d = 0; d < 3; d++) dmn[d] = mn[d], dmx[d] = mx[d];
+        __threadfence();
+        const int old = atomicAdd(&flags[pi], 1);
+        if (old == 0) return;
+        __threadfence();
+        const volatile Node *vn = nd;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float a = vn->lmin[d], b = vn->rmin[d];
+            const float c = vn->lmax[d], e = vn->rmax[d];
+            mn[d] = b < a ? b : a;
+            mx[d] = c < e ? e : c;
+        }
+        if (pi == 0) {
+            for (int d = 0; d < 3; d++) meta->root_box[d] = mn[d], meta->root_box[3 + d] = mx[d];
+            return;
+        }
+        p = parent[pi];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_depth(int n, const int *__restrict__ parent, MeshMeta *__restrict__ meta) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || n < 2) return;
+    int d = 0;
+    int p = parent[n - 1 + k];
+    for (;;) {
+        d++;
+        const int pi = p >> 1;
+        if (pi == 0) break;
+        if (d >= 4096) {  // not a tree: report an impossible depth instead of spinning
+            d = 1 << 20;
+            break;
+        }
+        p = parent[pi];
+    }
+    atomicMax(&meta->maxdepth, d);
+}
+
+static unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+int build_lbvh(DevMesh &m, hipStream_t stream) {
+    const int n = (int)m.nsel;
+    if (m.nsel <= 0 || m.nsel > 0x3fffffff) {
+        set_error("build_lbvh: triangle count out of range");
+        return SSDF_ERR_INVALID;
+    }
+    if (!m.meta) SSDF_HIP_TRY(hipMalloc(&m.meta, sizeof(MeshMeta)));
+    if (!m.nodes) SSDF_HIP_TRY(hipMalloc(&m.nodes, sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1)));
+    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriF) * (size_t)n));
+    m.device_bytes += sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1) + sizeof(TriF) * (size_t)n + sizeof(MeshMeta);
+    // workspace: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, flags 4 B per triangle
+    float *boxes = nullptr;
+    uint64_t *keys = nullptr;
+    uint32_t *vals = nullptr, *scratch = nullptr;
+    int *parent = nullptr, *flags = nullptr;
+    const size_t nn = (size_t)n;
+    SSDF_HIP_TRY(hipMalloc(&boxes, sizeof(float) * 6 * nn));
+    SSDF_HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * 2 * nn));
+    SSDF_HIP_TRY(hipMalloc(&vals, sizeof(uint32_t) * 2 * nn));
+    SSDF_HIP_TRY(hipMalloc(&parent, sizeof(int) * (2 * nn)));
+    SSDF_HIP_TRY(hipMalloc(&flags, sizeof(int) * (nn + 1)));
+    SSDF_HIP_TRY(hipMalloc(&scratch, sizeof(uint32_t) * radix_scratch_words(nn)));
+
+    hipLaunchKernelGGL(k_mesh_init, dim3(1), dim3(64), 0, stream, m.meta);
+    if (m.dtype == 0)
+        hipLaunchKernelGGL(k_tri_boxes<float>, dim3(nblk(nn, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
+                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, n, boxes, m.meta);
+    else
+        hipLaunchKernelGGL(k_tri_boxes<double>, dim3(nblk(nn, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
+                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, n, boxes, m.meta);
+    hipLaunchKernelGGL(k_morton, dim3(nblk(nn, 256)), dim3(256), 0, stream, boxes, n, m.meta, keys, vals);
+    uint64_t *skeys = nullptr;
+    uint32_t *svals = nullptr;
+    SSDF_HIP_TRY(radix_sort_pairs<uint64_t>(keys, keys + nn, vals, vals + nn, nn, 64, scratch, stream, &skeys, &svals));
+    if (n > 1)
+        hipLaunchKernelGGL(k_karras, dim3(nblk(nn - 1, 256)), dim3(256), 0, stream, skeys, n, m.nodes, parent, flags);
+    if (m.dtype == 0)
+        hipLaunchKernelGGL(k_refit_pack<float>, dim3(nblk(nn, 256)), dim3(256), 0, stream, svals, n, m.s0, m.s1,
+                           m.s2, (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, flags,
+                           m.nodes, m.tris, m.meta);
+    else
+        hipLaunchKernelGGL(k_refit_pack<double>, dim3(nblk(nn, 256)), dim3(256), 0, stream, svals, n, m.s0, m.s1,
+                           m.s2, (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent,
+                           flags, m.nodes, m.tris, m.meta);
+    hipLaunchKernelGGL(k_depth, dim3(nblk(nn, 256)), dim3(256), 0, stream, n, parent, m.meta);
+    SSDF_HIP_TRY(hipGetLastError());
+    // the frees below are stream-ordered only through the device-wide sync hipFree implies
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    (void)hipFree(boxes);
+    (void)hipFree(keys);
+    (void)hipFree(vals);
+    (void)hipFree(parent);
+    (void)hipFree(flags);
+    (void)hipFree(scratch);
+    return SSDF_OK;
+}
+
+int sync_meta(DevMesh &m, hipStream_t stream) {
+    SSDF_HIP_TRY(hipMemcpyAsync(&m.hmeta, m.meta, sizeof(MeshMeta), hipMemcpyDeviceToHost, stream));
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    return SSDF_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pseudonormals, FP64; follows serial/bvh/bvh.cpp:1669-1721.  The reference accumulates vertex
+// normals in triangle order and pairs face edges through a hash map; here both become stable
+// sorts, which visit the same contributions in the same order:
+//   vertex: entries 3t+v keyed by vertex id, summed per vertex in increasing t;
+//   edge:   entries 3t+e keyed min*V+max; the first face f0 of a key pairs with every later fk,
+//           edge_n[fk] = n[f0] + n[fk], and edge_n[f0] keeps the last sum.
+// ------------------------------------------------------------------------------------------------
+template <typename G>
+__global__ __launch_bounds__(256) void k_face_normals(const int *__restrict__ s0, const int *__restrict__ s1,
+                                                      const int *__restrict__ s2, const G *__restrict__ sx,
+                                                      const G *__restrict__ sy, const G *__restrict__ sz, int n,
+                                                      long long nsp, double *__restrict__ face_n,
+                                                      double *__restrict__ edge_n, double *__restrict__ angle,
+                                                      uint32_t *__restrict__ vkeys, uint32_t *__restrict__ vvals,
+                                                      uint64_t *__restrict__ ekeys, uint32_t *__restrict__ evals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int iv[3] = {s0[t], s1[t], s2[t]};
+    double p[3][3];
+#pragma unroll
+    for (int v = 0; v < 3; v++) p[v][0] = (double)sx[iv[v]], p[v][1] = (double)sy[iv[v]], p[v][2] = (double)sz[iv[v]];
+    const double u[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+    const double w[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+    double nn[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+    const double len = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    if (len > 0) {
+        for (int d = 0; d < 3; d++) nn[d] /= len;
+    }
+    for (int d = 0; d < 3; d++) face_n[(size_t)3 * t + d] = nn[d];
+    for (int e = 0; e < 3; e++)
+        for (int d = 0; d < 3; d++) edge_n[(size_t)9 * t + 3 * e + d] = nn[d];
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        const double *a = p[v], *b = p[(v + 1) % 3], *c = p[(v + 2) % 3];
+        const double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+        const double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        const double cosine = (l1 > 0 && l2 > 0) ? (e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2]) / (l1 * l2) : 1.0;
+        angle[(size_t)3 * t + v] = acos(fmax(-1.0, fmin(1.0, cosine)));
+        vkeys[(size_t)3 * t + v] = (uint32_t)iv[v];
+        vvals[(size_t)3 * t + v] = (uint32_t)(3 * t + v);
+        const uint64_t ea = (uint64_t)iv[v], eb = (uint64_t)iv[(v + 1) % 3];
+        ekeys[(size_t)3 * t + v] = (ea < eb ? ea : eb) * (uint64_t)nsp + (ea < eb ? eb : ea);
+        evals[(size_t)3 * t + v] = (uint32_t)(3 * t + v);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_f64(double *__restrict__ p, size_t n, double v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_vertex_normals(const uint32_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ vals, size_t m,
+                                                        const double *__restrict__ face_n,
+                                                        const double *__restrict__ angle,
+                                                        double *__restrict__ vertex_n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t key = keys[i];
+    if (i > 0 && keys[i - 1] == key) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t j = i; j < m && keys[j] == key; j++) {
+        const uint32_t e = vals[j];
+        const double a = angle[e];
+        const size_t t = e / 3;
+        for (int d = 0; d < 3; d++) acc[d] += a * face_n[3 * t + d];
+    }
+    for (int d = 0; d < 3; d++) vertex_n[(size_t)3 * key + d] = acc[d];
+}
+
+__global__ __launch_bounds__(256) void k_edge_normals(const uint64_t *__restrict__ keys,
+                                                      const uint32_t *__restrict__ vals, size_t m,
+                                                      const int *__restrict__ s0, const int *__restrict__ s1,
+                                                      const int *__restrict__ s2, const double *__restrict__ face_n,
+                                                      double *__restrict__ edge_n, MeshMeta *__restrict__ meta) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t key = keys[i];
+    if (i > 0 && keys[i - 1] == key) return;
+    size_t j = i + 1;
+    while (j < m && keys[j] == key) j++;
+    const size_t g = j - i;
+    const uint32_t f0 = vals[i];
+    const size_t t0 = f0 / 3;
+    for (size_t k = i + 1; k < j; k++) {
+        const uint32_t fk = vals[k];
+        const size_t tk = fk / 3;
+        for (int d = 0; d < 3; d++) {
+            const double sum = face_n[3 * t0 + d] + face_n[3 * tk + d];
+            edge_n[9 * tk + 3 * (fk % 3) + d] = sum;
+            edge_n[9 * t0 + 3 * (f0 % 3) + d] = sum;
+        }
+    }
+    if (g != 2) {
+        atomicAdd(&meta->open_edges, 1);
+    } else {
+        const int *s[3] = {s0, s1, s2};
+        const uint32_t f1 = vals[i + 1];
+        const int a0 = s[f0 % 3][t0], b0 = s[(f0 % 3 + 1) % 3][t0];
+        const int a1 = s[f1 % 3][f1 / 3];
+        if (a0 == a1 && b0 != a0) atomicAdd(&meta->flipped_edges, 1);
+    }
+}
+
+__global__ void k_meta_zero_edges(MeshMeta *meta) {
+    meta->open_edges = 0;
+    meta->flipped_edges = 0;
+}
+
+int build_normals(DevMesh &m, hipStream_t stream) {
+    const size_t n = (size_t)m.nsel, V = (size_t)m.nsp;
+    if (m.has_normals) return SSDF_OK;
+    if (!m.face_n) SSDF_HIP_TRY(hipMalloc(&m.face_n, sizeof(double) * 3 * n));
+    if (!m.edge_n) SSDF_HIP_TRY(hipMalloc(&m.edge_n, sizeof(double) * 9 * n));
+    if (!m.vertex_n) SSDF_HIP_TRY(hipMalloc(&m.vertex_n, sizeof(double) * 3 * V));
+    m.device_bytes += sizeof(double) * (12 * n + 3 * V);
+    const size_t e = 3 * n;
+    double *angle = nullptr;
+    uint32_t *vkeys = nullptr, *vvals = nullptr, *evals = nullptr, *scratch = nullptr;
+    uint64_t *ekeys = nullptr;
+    SSDF_HIP_TRY(hipMalloc(&angle, sizeof(double) * e));
+    SSDF_HIP_TRY(hipMalloc(&vkeys, sizeof(uint32_t) * 2 * e));
+    SSDF_HIP_TRY(hipMalloc(&vvals, sizeof(uint32_t) * 2 * e));
+    SSDF_HIP_TRY(hipMalloc(&ekeys, sizeof(uint64_t) * 2 * e));
+    SSDF_HIP_TRY(hipMalloc(&evals, sizeof(uint32_t) * 2 * e));
+    SSDF_HIP_TRY(hipMalloc(&scratch, sizeof(uint32_t) * radix_scratch_words(e)));
+    hipLaunchKernelGGL(k_meta_zero_edges, dim3(1), dim3(1), 0, stream, m.meta);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * V, 256)), dim3(256), 0, stream, m.vertex_n, 3 * V, 0.0);
+    if (m.dtype == 0)
+        hipLaunchKernelGGL(k_face_normals<float>, dim3(nblk(n, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
+                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, (int)n, (long long)V,
+                           m.face_n, m.edge_n, angle, vkeys, vvals, ekeys, evals);
+    else
+        hipLaunchKernelGGL(k_face_normals<double>, dim3(nblk(n, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
+                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, (int)n, (long long)V,
+                           m.face_n, m.edge_n, angle, vkeys, vvals, ekeys, evals);
+    int vbits = 1;
+    while (vbits < 32 && ((size_t)1 << vbits) < V) vbits++;
+    int ebits = 1;
+    const unsigned long long maxkey = (unsigned long long)V * (unsigned long long)V;
+    while (ebits < 64 && (1ull << ebits) < maxkey) ebits++;
+    uint32_t *svk = nullptr, *svv = nullptr, *sev = nullptr;
+    uint64_t *sek = nullptr;
+    SSDF_HIP_TRY(radix_sort_pairs<uint32_t>(vkeys, vkeys + e, vvals, vvals + e, e, vbits, scratch, stream, &svk, &svv));
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, svk, svv, e, m.face_n, angle,
+                       m.vertex_n);
+    SSDF_HIP_TRY(radix_sort_pairs<uint64_t>(ekeys, ekeys + e, evals, evals + e, e, ebits, scratch, stream, &sek, &sev));
+    hipLaunchKernelGGL(k_edge_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, sek, sev, e, m.s0, m.s1, m.s2,
+                       m.face_n, m.edge_n, m.meta);
+    SSDF_HIP_TRY(hipGetLastError());
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    (void)hipFree(angle);
+    (void)hipFree(vkeys);
+    (void)hipFree(vvals);
+    (void)hipFree(ekeys);
+    (void)hipFree(evals);
+    (void)hipFree(scratch);
+    m.has_normals = true;
+    return SSDF_OK;
+}
+
+void free_mesh(DevMesh &m) {
+    if (m.owns_input) {
+        (void)hipFree(m.s0);
+        (void)hipFree(m.s1);
+        (void)hipFree(m.s2);
+        (void)hipFree(m.sx);
+        (void)hipFree(m.sy);
+        (void)hipFree(m.sz);
+    }
+    (void)hipFree(m.nodes);
+    (void)hipFree(m.tris);
+    (void)hipFree(m.meta);
+    (void)hipFree(m.face_n);
+    (void)hipFree(m.edge_n);
+    (void)hipFree(m.vertex_n);
+    m = DevMesh{};
+}
+
+}  // namespace ssdf_amd

@@ -1,0 +1,61 @@
+// mesh.hpp — a triangle mesh resident in HBM with its LBVH and (optionally) its pseudonormals.
+#pragma once
+
+#include "common.hpp"
+
+#include <string>
+
+namespace ssdf_amd {
+
+// Per-mesh scalars the kernels read from device memory (so that a captured build + query never
+// needs a host round trip).
+struct MeshMeta {
+    int root;       // child reference of the root (internal 0, or leaf ~0 for one triangle)
+    int maxdepth;   // deepest leaf, edges from the root
+    int bbox_ord[6];  // vertex bounds as ordered ints (min xyz, max xyz)
+    int cbox_ord[6];  // triangle-centre bounds as ordered ints
+    float root_box[6];
+    int open_edges;  // edges not shared by exactly two faces (-1 until normals are built)
+    int flipped_edges;  // manifold edges whose two faces traverse it in the same direction
+};
+
+struct DevMesh {
+    int dtype = 0;  // 0: float vertices, 1: double vertices
+    ptrdiff_t nsel = 0, nsp = 0;
+    // caller-layout copies (SoA), needed for FP64 re-evaluation and normals
+    int *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+    void *sx = nullptr, *sy = nullptr, *sz = nullptr;
+    bool owns_input = true;
+    // LBVH
+    Node *nodes = nullptr;  // nsel - 1 (at least 1)
+    TriF *tris = nullptr;   // nsel, leaf order
+    MeshMeta *meta = nullptr;  // device
+    MeshMeta hmeta{};          // host copy after sync_meta()
+    // pseudonormals (FP64): face 3T, edge 9T, vertex 3V
+    double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
+    bool has_normals = false;
+    size_t device_bytes = 0;
+};
+
+// Uploads nothing: builds the LBVH of a mesh whose SoA arrays are already on the device.
+int build_lbvh(DevMesh &m, hipStream_t stream);
+// FP64 pseudonormals on the device (face, edge, vertex), following serial/bvh/bvh.cpp:1669-1721.
+int build_normals(DevMesh &m, hipStream_t stream);
+// Copies meta back (synchronises the stream).
+int sync_meta(DevMesh &m, hipStream_t stream);
+void free_mesh(DevMesh &m);
+
+// thread-local last error message of the C ABI
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define SSDF_HIP_TRY(expr)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::ssdf_amd::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return e_ == hipErrorOutOfMemory ? SSDF_ERR_OOM : SSDF_ERR_HIP; \
+        }                                                                                     \
+    } while (0)
+
+}  // namespace ssdf_amd

@@ -1,0 +1,630 @@
+// query.hip — closest-triangle queries on a resident LBVH.
+//
+// k_query_packet: wave-coherent traversal.  A wave (64 queries: a 4x4x4 brick of grid nodes, or 64
+// consecutive points) walks the tree as one: the node index is wave-uniform, so each 64-byte node
+// record arrives with one scalar load, every lane tests its own query against both child boxes,
+// and two ballots decide whether to descend left, right or both (nearest-first by majority vote,
+// the far child's box pushed on a wave-uniform LDS stack).  Popped entries are re-tested against
+// each lane's current bound before the node is fetched again.  Leaves hold one triangle; the whole
+// wave evaluates it (point_triangle_dist_sq, serial/ssdf.hpp:50-121, in FP32).
+//
+// FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
+// FP32 error bound of a distance, and each lane keeps the kCand nearest triangles inside that band.
+// After traversal, a lane whose band holds one triangle is final; two or more are re-evaluated in
+// FP64 (same formula, contraction off -> the oracle's bits) and the lowest index wins exact ties;
+// a lane whose list overflowed is queued for k_query_exact (full FP64 traversal).
+//
+// k_query_exact: one lane per query, FP64 box and triangle tests, private stack; exact argmin with
+// the canonical tie-break.  Used for FP64 meshes and for the fixup queue.
+#include "query.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace ssdf_amd {
+
+constexpr int kWaves = 4;
+constexpr int kCand = 8;
+constexpr int kNone = INT_MIN;
+
+struct alignas(16) StackEnt {
+    float mn[3];
+    int node;
+    float mx[3];
+    int pad;
+};
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// largest float below x (x > 0), so that "y <= prev_f(x)" is "y < x"
+__device__ __forceinline__ float prev_f(float x) {
+    if (x > 0.0f) return __int_as_float(__float_as_int(x) - 1);
+    if (x == 0.0f) return -1.401298464e-45f;
+    return x;
+}
+
+__device__ __forceinline__ float band_sq(float d2, float tau1, float babs) {
+    const float r = sqrtf(d2) * tau1 + babs;
+    return r * r;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_as(const void *p, long long i);
+template <>
+__device__ __forceinline__ float load_as<float>(const void *p, long long i) {
+    return ((const float *)p)[i];
+}
+template <>
+__device__ __forceinline__ double load_as<double>(const void *p, long long i) {
+    return ((const double *)p)[i];
+}
+
+// Query of a lane.  Grid: waves map to 4x4x4 bricks, bricks to 4x4x4 super-bricks in Morton order
+// (spatially compact waves and blocks), lanes to (lane&3, lane>>2&3, lane>>4).  Points: 64
+// consecutive points per wave.
+template <int QSRC, typename Q>
+__device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id, int lane, long long &qid, Q &x,
+                                           Q &y, Q &z) {
+    if (QSRC == QSRC_POINTS) {
+        qid = (long long)wave_id * 64 + lane;
+        if (qid >= a.npoints) return false;
+        x = load_as<Q>(a.qx, qid), y = load_as<Q>(a.qy, qid), z = load_as<Q>(a.qz, qid);
+        return true;
+    } else {
+        const unsigned sb = wave_id >> 6, l = wave_id & 63u;
+        const unsigned nsx = (a.nbx + 3) >> 2, nsy = (a.nby + 3) >> 2;
+        const unsigned sbx = sb % nsx, sby = (sb / nsx) % nsy, sbz = sb / (nsx * nsy);
+        const unsigned lx = (l & 1u) | ((l >> 2) & 2u), ly = ((l >> 1) & 1u) | ((l >> 3) & 2u),
+                       lz = ((l >> 2) & 1u) | ((l >> 4) & 2u);
+        const int i = (int)((sbx * 4 + lx) * 4) + (lane & 3);
+        const int j = (int)((sby * 4 + ly) * 4) + ((lane >> 2) & 3);
+        const int k = (int)((sbz * 4 + lz) * 4) + (lane >> 4);
+        if (i >= a.nx || j >= a.ny || k >= a.nz) return false;
+        qid = (long long)i + (long long)a.nx * ((long long)j + (long long)a.ny * k);
+        const Q hx = (Q)a.gh[0], hy = (Q)a.gh[1], hz = (Q)a.gh[2];
+        x = (Q)a.gmin[0] + (Q)i * hx;
+        y = (Q)a.gmin[1] + (Q)j * hy;
+        z = (Q)a.gmin[2] + (Q)k * hz;
+        return true;
+    }
+}
+
+// Sign of (q - c).n at the closest point c of triangle (a, b, c_); region by barycentrics with the
+// reference's tolerance; follows serial/bvh/bvh.cpp:1734-1792.  Returns the dot product and |n|.
+template <typename T>
+__device__ __forceinline__ T sign_dot(const T qx, const T qy, const T qz, const T *A, const T *B, const T *C, int t,
+                                      const int *__restrict__ s0, const int *__restrict__ s1,
+                                      const int *__restrict__ s2, const double *__restrict__ face_n,
+                                      const double *__restrict__ edge_n, const double *__restrict__ vertex_n,
+                                      T &nlen) {
+    T c[3];
+    pt_tri_cp<T>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2], c[0], c[1], c[2]);
+    const T e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    const T e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    const T r[3] = {c[0] - A[0], c[1] - A[1], c[2] - A[2]};
+    const T d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const T d12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+    const T d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    const T r1 = r[0] * e1[0] + r[1] * e1[1] + r[2] * e1[2];
+    const T r2 = r[0] * e2[0] + r[1] * e2[1] + r[2] * e2[2];
+    const T det = d11 * d22 - d12 * d12;
+    const T l1 = det > T(0) ? (d22 * r1 - d12 * r2) / det : T(0);
+    const T l2 = det > T(0) ? (d11 * r2 - d12 * r1) / det : T(0);
+    const T l[3] = {T(1) - l1 - l2, l1, l2};
+    const T tol = T(1e-4);
+    int small = 0, last_small = -1, last_large = 0;
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        if (l[v] < tol)
+            small++, last_small = v;
+        else
+            last_large = v;
+    }
+    const double *n;
+    if (small == 0) {
+        n = face_n + (size_t)3 * t;
+    } else if (small == 1) {
+        n = edge_n + (size_t)9 * t + 3 * ((last_small + 1) % 3);
+    } else {
+        const int vid = last_large == 0 ? s0[t] : (last_large == 1 ? s1[t] : s2[t]);
+        n = vertex_n + (size_t)3 * vid;
+    }
+    const T n0 = (T)n[0], n1 = (T)n[1], n2 = (T)n[2];
+    nlen = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    return (qx - c[0]) * n0 + (qy - c[1]) * n1 + (qz - c[2]) * n2;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_as(void *p, long long i, T v) {
+    ((T *)p)[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Wave-coherent FP32 traversal (float queries, float meshes).
+// ------------------------------------------------------------------------------------------------
+template <int STACK, int QSRC, int OUT, bool FIX>
+__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriF *__restrict__ tris,
+                                                      QueryArgs a) {
+    __shared__ StackEnt stk_all[kWaves][STACK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    StackEnt *stk = stk_all[w];
+    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
+    const unsigned wave_id = blk * kWaves + (unsigned)w;
+    long long qid = 0;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    const bool valid = lane_query<QSRC, float>(a, wave_id, lane, qid, qx, qy, qz);
+    if (__ballot(valid) == 0) return;
+
+    // FP32 error bound of a distance: 2^-18 (32 ulps of 1) times the largest coordinate involved
+    const float tau1 = 1.0f + a.tau;
+    const float babs = fmaxf(a.band_abs, 0x1p-18f * fmaxf(fabsf(qx), fmaxf(fabsf(qy), fabsf(qz))));
+
+    // strict upper bound from the caller's current distance (EDF / CLOSEST), inclusive form
+    float init_thr = SSDF_INF;
+    float out_init = SSDF_INF;
+    if ((OUT == OUT_EDF || OUT == OUT_CLOSEST) && valid) {
+        out_init = ((const float *)a.out_d)[qid];
+        init_thr = prev_f(out_init * out_init);
+    }
+    float thr = valid ? init_thr : -SSDF_INF;
+
+    float bd = SSDF_INF;
+    int bk = -1;
+    float cd[kCand];
+    int ck[kCand];
+    float dropped = SSDF_INF;
+    if (FIX) {
+#pragma unroll
+        for (int i = 0; i < kCand; i++) cd[i] = SSDF_INF, ck[i] = -1;
+    }
+
+    int node = uniform_i(a.meta->root);
+    int sp = 0;
+    long long iters = 0;
+    for (;;) {
+        if (++iters > a.max_iters) {
+            if (lane == 0) atomicOr(a.err_flag, 2);
+            break;
+        }
+        if (node >= 0) {
+            const Node nd = nodes[node];
+            const float dl = pt_box_d2<float>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
+                                              nd.lmax[1], nd.lmax[2]);
+            const float dr = pt_box_d2<float>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
+                                              nd.rmax[1], nd.rmax[2]);
+            const bool nl = dl <= thr, nr = dr <= thr;
+            const uint64_t ml = __ballot(nl), mr = __ballot(nr);
+            if (ml != 0 && mr != 0) {
+                const uint64_t votel = __ballot(nl && (!nr || dl <= dr));
+                const bool lfirst = 2 * __popcll(votel) >= __popcll(ml | mr);
+                const int nearc = lfirst ? nd.left : nd.right;
+                if (sp < STACK) {
+                    if (lane == 0) {
+                        StackEnt e;
+                        e.node = lfirst ? nd.right : nd.left;
+                        e.pad = 0;
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            e.mn[d] = lfirst ? nd.rmin[d] : nd.lmin[d];
+                            e.mx[d] = lfirst ? nd.rmax[d] : nd.lmax[d];
+                        }
+                        stk[sp] = e;
+                    }
+                    sp++;
+                } else if (lane == 0) {
+                    atomicOr(a.err_flag, 1);
+                }
+                node = uniform_i(nearc);
+                continue;
+            }
+            if (ml != 0) {
+                node = uniform_i(nd.left);
+                continue;
+            }
+            if (mr != 0) {
+                node = uniform_i(nd.right);
+                continue;
+            }
+        } else {
+            const int k = ~node;
+            const TriF tr = tris[k];
+            const float d2 = pt_tri_d2<float>(qx, qy, qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2],
+                                              tr.c[0], tr.c[1], tr.c[2]);
+            if (!FIX) {
+                if (d2 <= thr) {
+                    bd = d2;
+                    bk = k;
+                    thr = prev_f(d2);
+                }
+            } else {
+                const bool ins = d2 <= thr;
+                if (__ballot(ins) != 0) {
+                    if (ins) {
+                        float xd = d2;
+                        int xk = k;
+#pragma unroll
+                        for (int i = 0; i < kCand; i++) {
+                            const bool sw = xd < cd[i];
+                            const float td = cd[i];
+                            const int tk = ck[i];
+                            cd[i] = sw ? xd : td;
+                            ck[i] = sw ? xk : tk;
+                            xd = sw ? td : xd;
+                            xk = sw ? tk : xk;
+                        }
+                        dropped = fminf(dropped, xd);
+                        thr = fminf(band_sq(cd[0], tau1, babs), init_thr);
+                    }
+                }
+            }
+        }
+        // pop the nearest pending subtree some lane still needs
+        node = kNone;
+        while (sp > 0) {
+            --sp;
+            const StackEnt e = stk[sp];
+            const float dd = pt_box_d2<float>(qx, qy, qz, e.mn[0], e.mn[1], e.mn[2], e.mx[0], e.mx[1], e.mx[2]);
+            if (__ballot(dd <= thr) != 0) {
+                node = uniform_i(e.node);
+                break;
+            }
+        }
+        if (node == kNone) break;
+    }
+    if (!valid) return;
+
+    // ---- resolve the closest triangle ----
+    int k_final = -1;      // leaf position
+    bool have64 = false;   // distance known in FP64
+    double d2_64 = 0.0;
+    float d2_32 = SSDF_INF;
+    if (!FIX) {
+        k_final = bk;
+        d2_32 = bd;
+    } else {
+        const float bt = fminf(band_sq(cd[0], tau1, babs), init_thr);
+        int nb = 0;
+#pragma unroll
+        for (int i = 0; i < kCand; i++) nb += (ck[i] >= 0 && cd[i] <= bt) ? 1 : 0;
+        k_final = nb > 0 ? ck[0] : -1;
+        d2_32 = nb > 0 ? cd[0] : SSDF_INF;
+        if (dropped < SSDF_INF && dropped <= bt) {
+            // the exact kernel recomputes this query from the untouched output (its seed)
+            const unsigned pos = atomicAdd(a.fix_count, 1u);
+            if (pos < a.fix_cap) a.fix_list[pos] = (unsigned)qid;
+            return;
+        }
+        if (nb >= 2) {
+            const double px = qx, py = qy, pz = qz;
+            double bestd = __builtin_inf();
+            int bestt = INT_MAX, bestk = -1;
+#pragma unroll
+            for (int i = 0; i < kCand; i++) {
+                if (ck[i] >= 0 && cd[i] <= bt) {
+                    const TriF tr = tris[ck[i]];
+                    const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
+                                                       tr.b[2], tr.c[0], tr.c[1], tr.c[2]);
+                    if (d < bestd || (d == bestd && tr.orig < bestt)) bestd = d, bestt = tr.orig, bestk = ck[i];
+                }
+            }
+            if (bestk >= 0) {
+                k_final = bestk;
+                have64 = true;
+                d2_64 = bestd;
+            }
+        }
+    }
+
+    const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(d2_32);
+    if (OUT == OUT_EDF) {
+        if (k_final >= 0) {
+            const float o = fminf(out_init, dist);
+            ((float *)a.out_d)[qid] = o;
+            if (a.out_tri) a.out_tri[qid] = tris[k_final].orig;
+        }
+        return;
+    }
+    if (k_final < 0) {
+        if (OUT == OUT_SDF) {
+            ((float *)a.out_d)[qid] = SSDF_INF;
+            if (a.out_tri) a.out_tri[qid] = -1;
+        }
+        return;
+    }
+    const TriF tr = tris[k_final];
+    if (OUT == OUT_CLOSEST) {
+        if (out_init > dist) {
+            float cx, cy, cz;
+            if (have64) {
+                double ex, ey, ez;
+                pt_tri_cp<double>((double)qx, (double)qy, (double)qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
+                                  tr.b[2], tr.c[0], tr.c[1], tr.c[2], ex, ey, ez);
+                cx = (float)ex, cy = (float)ey, cz = (float)ez;
+            } else {
+                pt_tri_cp<float>(qx, qy, qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2], tr.c[0], tr.c[1],
+                                 tr.c[2], cx, cy, cz);
+            }
+            ((float *)a.out_d)[qid] = dist;
+            ((float *)a.out_cx)[qid] = cx;
+            ((float *)a.out_cy)[qid] = cy;
+            ((float *)a.out_cz)[qid] = cz;
+            if (a.out_tri) a.out_tri[qid] = tr.orig;
+        }
+        return;
+    }
+    // OUT_SDF: sign from the pseudonormal at the closest point; FP64 near the surface or when the
+    // FP32 dot product is too small to trust
+    float nlen32;
+    const float dot32 = sign_dot<float>(qx, qy, qz, tr.a, tr.b, tr.c, tr.orig, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
+                                        a.vertex_n, nlen32);
+    bool neg = dot32 < 0.0f;
+    if (dist < a.sign_eps || fabsf(dot32) <= (1e-3f * dist + 4.0f * babs) * nlen32) {
+        const double A[3] = {tr.a[0], tr.a[1], tr.a[2]}, B[3] = {tr.b[0], tr.b[1], tr.b[2]},
+                     C[3] = {tr.c[0], tr.c[1], tr.c[2]};
+        double nl64;
+        const double dot64 = sign_dot<double>((double)qx, (double)qy, (double)qz, A, B, C, tr.orig, a.s0, a.s1,
+                                              a.s2, a.face_n, a.edge_n, a.vertex_n, nl64);
+        neg = dot64 < 0.0;
+    }
+    ((float *)a.out_d)[qid] = neg ? -dist : dist;
+    if (a.out_tri) a.out_tri[qid] = tr.orig;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Exact FP64 traversal, one lane per query.  A box is skipped only when its FP64 distance exceeds
+// the best distance by delta64 (far above FP64 rounding of any distance), so the result is the
+// brute-force argmin of the FP64 squared distance with the lowest index on exact ties.
+// ------------------------------------------------------------------------------------------------
+template <typename G>
+__device__ __forceinline__ void tri_verts64(const QueryArgs &a, const TriF &tr, double *A, double *B, double *C) {
+    if (sizeof(G) == sizeof(float)) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) A[d] = tr.a[d], B[d] = tr.b[d], C[d] = tr.c[d];
+    } else {
+        const int t = tr.orig;
+        const int i0 = a.s0[t], i1 = a.s1[t], i2 = a.s2[t];
+        const double *sx = (const double *)a.sx, *sy = (const double *)a.sy, *sz = (const double *)a.sz;
+        A[0] = sx[i0], A[1] = sy[i0], A[2] = sz[i0];
+        B[0] = sx[i1], B[1] = sy[i1], B[2] = sz[i1];
+        C[0] = sx[i2], C[1] = sy[i2], C[2] = sz[i2];
+    }
+}
+
+constexpr int kExactStack = 128;
+
+// QSRC: points / grid, or -1 for the fixup queue (query ids in fix_list, points or grid by GSRC).
+template <typename G, int QSRC, int OUT, bool QUEUE>
+__global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
+    long long qid;
+    double qx, qy, qz;
+    const unsigned n_items = QUEUE ? min(*a.fix_count, a.fix_cap) : 0u;
+    const long long total = QUEUE ? (long long)n_items
+                                  : (QSRC == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz);
+    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+         item += (long long)gridDim.x * blockDim.x) {
+        if (QUEUE) {
+            qid = (long long)a.fix_list[item];
+        } else {
+            qid = item;
+        }
+        if (QSRC == QSRC_POINTS) {
+            qx = (double)load_as<G>(a.qx, qid), qy = (double)load_as<G>(a.qy, qid), qz = (double)load_as<G>(a.qz, qid);
+        } else {
+            const long long i = qid % a.nx, j = (qid / a.nx) % a.ny, k = qid / ((long long)a.nx * a.ny);
+            const G hx = (G)a.gh[0], hy = (G)a.gh[1], hz = (G)a.gh[2];
+            qx = (double)((G)a.gmin[0] + (G)i * hx);
+            qy = (double)((G)a.gmin[1] + (G)j * hy);
+            qz = (double)((G)a.gmin[2] + (G)k * hz);
+        }
+        double out_init = __builtin_inf();
+        if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
+        const double init2 = out_init * out_init;
+        double best = __builtin_inf(), bound = __builtin_inf();
+        int bt = INT_MAX, bk = -1;
+        const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
+        {
+            const double b0 = sqrt(init2) + delta;
+            bound = b0 * b0;
+        }
+        int st_node[kExactStack];
+        double st_d[kExactStack];
+        int sp = 0;
+        int node = a.meta->root;
+        long long iters = 0;
+        for (;;) {
+            if (++iters > a.max_iters) {
+                atomicOr(a.err_flag, 2);
+                break;
+            }
+            if (node >= 0) {
+                const Node nd = a.nodes[node];
+                const double dl = pt_box_d2<double>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
+                                                    nd.lmax[1], nd.lmax[2]);
+                const double dr = pt_box_d2<double>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
+                                                    nd.rmax[1], nd.rmax[2]);
+                const bool nl = dl <= bound, nr = dr <= bound;
+                if (nl && nr) {
+                    const bool lfirst = dl <= dr;
+                    if (sp < kExactStack) {
+                        st_node[sp] = lfirst ? nd.right : nd.left;
+                        st_d[sp] = lfirst ? dr : dl;
+                        sp++;
+                    } else {
+                        atomicOr(a.err_flag, 1);
+                    }
+                    node = lfirst ? nd.left : nd.right;
+                    continue;
+                }
+                if (nl) {
+                    node = nd.left;
+                    continue;
+                }
+                if (nr) {
+                    node = nd.right;
+                    continue;
+                }
+            } else {
+                const TriF tr = a.tris[~node];
+                double A[3], B[3], C[3];
+                tri_verts64<G>(a, tr, A, B, C);
+                const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
+                if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
+                    best = d, bt = tr.orig, bk = ~node;
+                    const double r = sqrt(best) + delta;
+                    bound = r * r;
+                }
+            }
+            node = kNone;
+            while (sp > 0) {
+                --sp;
+                if (st_d[sp] <= bound) {
+                    node = st_node[sp];
+                    break;
+                }
+            }
+            if (node == kNone) break;
+        }
+        // outputs in caller precision G
+        const double dist = sqrt(best);
+        if (OUT == OUT_EDF) {
+            if (bk >= 0) {
+                store_as<G>(a.out_d, qid, (G)fmin(out_init, dist));
+                if (a.out_tri) a.out_tri[qid] = bt;
+            }
+            continue;
+        }
+        if (bk < 0) {
+            if (OUT == OUT_SDF) {
+                store_as<G>(a.out_d, qid, (G)__builtin_inf());
+                if (a.out_tri) a.out_tri[qid] = -1;
+            }
+            continue;
+        }
+        const TriF tr = a.tris[bk];
+        double A[3], B[3], C[3];
+        tri_verts64<G>(a, tr, A, B, C);
+        if (OUT == OUT_CLOSEST) {
+            const G gd = (G)dist;
+            if ((G)out_init > gd) {
+                double cx, cy, cz;
+                pt_tri_cp<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2], cx, cy, cz);
+                store_as<G>(a.out_d, qid, gd);
+                store_as<G>(a.out_cx, qid, (G)cx);
+                store_as<G>(a.out_cy, qid, (G)cy);
+                store_as<G>(a.out_cz, qid, (G)cz);
+                if (a.out_tri) a.out_tri[qid] = bt;
+            }
+            continue;
+        }
+        double nl64;
+        const double dot = sign_dot<double>(qx, qy, qz, A, B, C, bt, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
+                                            a.vertex_n, nl64);
+        store_as<G>(a.out_d, qid, (G)(dot < 0 ? -dist : dist));
+        if (a.out_tri) a.out_tri[qid] = bt;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host dispatch
+// ------------------------------------------------------------------------------------------------
+template <int STACK, int QSRC, int OUT, bool FIX>
+static void launch_packet(QueryArgs &a, unsigned nblocks, hipStream_t s) {
+    a.nblocks = nblocks;
+    hipLaunchKernelGGL((k_query_packet<STACK, QSRC, OUT, FIX>), dim3(nblocks), dim3(256), 0, s, a.nodes, a.tris, a);
+}
+
+template <int STACK, int QSRC>
+static int dispatch_packet_out(QueryArgs &a, int out, bool fix, unsigned nblocks, hipStream_t s) {
+    if (out == OUT_EDF) {
+        fix ? launch_packet<STACK, QSRC, OUT_EDF, true>(a, nblocks, s)
+            : launch_packet<STACK, QSRC, OUT_EDF, false>(a, nblocks, s);
+    } else if (out == OUT_SDF) {
+        fix ? launch_packet<STACK, QSRC, OUT_SDF, true>(a, nblocks, s)
+            : launch_packet<STACK, QSRC, OUT_SDF, false>(a, nblocks, s);
+    } else {
+        if (QSRC != QSRC_POINTS) return SSDF_ERR_UNSUPPORTED;
+        fix ? launch_packet<STACK, QSRC_POINTS, OUT_CLOSEST, true>(a, nblocks, s)
+            : launch_packet<STACK, QSRC_POINTS, OUT_CLOSEST, false>(a, nblocks, s);
+    }
+    return SSDF_OK;
+}
+
+template <typename G, int QSRC, bool QUEUE>
+static int dispatch_exact_out(QueryArgs &a, int out, unsigned nblocks, hipStream_t s) {
+    if (out == OUT_EDF)
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_EDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+    else if (out == OUT_SDF)
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_CLOSEST, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+    return SSDF_OK;
+}
+
+int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,
+                 int q_is_double, hipStream_t stream) {
+    if (m.nsel <= 0) return SSDF_OK;
+    if (out == OUT_SDF && !m.has_normals) {
+        set_error("signed queries need the mesh's pseudonormals");
+        return SSDF_ERR_INVALID;
+    }
+    a.nodes = m.nodes;
+    a.tris = m.tris;
+    a.meta = m.meta;
+    a.s0 = m.s0, a.s1 = m.s1, a.s2 = m.s2;
+    a.sx = m.sx, a.sy = m.sy, a.sz = m.sz;
+    a.face_n = m.face_n, a.edge_n = m.edge_n, a.vertex_n = m.vertex_n;
+    a.max_iters = 4 * (long long)m.nsel + 64;
+    const long long nq = qsrc == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz;
+    if (nq <= 0) return SSDF_OK;
+    if (m.dtype == 1) {
+        // FP64 mesh: FP64 queries and outputs, exact traversal
+        if (!(out_is_double && q_is_double)) {
+            set_error("FP64 meshes take FP64 queries and FP64 outputs");
+            return SSDF_ERR_UNSUPPORTED;
+        }
+        const unsigned nb = (unsigned)std::min<long long>((nq + 63) / 64, 1 << 20);
+        const int rc = qsrc == QSRC_POINTS ? dispatch_exact_out<double, QSRC_POINTS, false>(a, out, nb, stream)
+                                           : dispatch_exact_out<double, QSRC_GRID, false>(a, out, nb, stream);
+        SSDF_HIP_TRY(hipGetLastError());
+        return rc;
+    }
+    if (out_is_double || q_is_double) {
+        set_error("FP32 meshes take FP32 queries and FP32 outputs");
+        return SSDF_ERR_UNSUPPORTED;
+    }
+    const bool fix = (flags & QF_FIXUP) != 0;
+    const int depth = m.hmeta.maxdepth;
+    unsigned nwaves;
+    if (qsrc == QSRC_POINTS) {
+        nwaves = (unsigned)((nq + 63) / 64);
+    } else {
+        a.nbx = (a.nx + 3) / 4, a.nby = (a.ny + 3) / 4, a.nbz = (a.nz + 3) / 4;
+        const unsigned nsx = (a.nbx + 3) / 4, nsy = (a.nby + 3) / 4, nsz = (a.nbz + 3) / 4;
+        nwaves = nsx * nsy * nsz * 64;
+    }
+    const unsigned nblocks = (nwaves + kWaves - 1) / kWaves;
+    int rc;
+    if (depth <= 64) {
+        rc = qsrc == QSRC_POINTS ? dispatch_packet_out<64, QSRC_POINTS>(a, out, fix, nblocks, stream)
+                                 : dispatch_packet_out<64, QSRC_GRID>(a, out, fix, nblocks, stream);
+    } else if (depth <= 128) {
+        rc = qsrc == QSRC_POINTS ? dispatch_packet_out<128, QSRC_POINTS>(a, out, fix, nblocks, stream)
+                                 : dispatch_packet_out<128, QSRC_GRID>(a, out, fix, nblocks, stream);
+    } else {
+        set_error("LBVH deeper than 128 levels");
+        return SSDF_ERR_DEPTH;
+    }
+    if (rc) return rc;
+    if (fix) {
+        // queued overflow lanes: exact FP64 traversal, grid-stride over the device-side count
+        const unsigned nb = 1024;
+        if (qsrc == QSRC_POINTS)
+            rc = dispatch_exact_out<float, QSRC_POINTS, true>(a, out, nb, stream);
+        else
+            rc = dispatch_exact_out<float, QSRC_GRID, true>(a, out, nb, stream);
+    }
+    SSDF_HIP_TRY(hipGetLastError());
+    return rc;
+}
+
+}  // namespace ssdf_amd

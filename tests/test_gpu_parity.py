@@ -1,0 +1,300 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same seeded inputs.
+
+Bars (from the engine's precision contract):
+  FP64 path (ssdf_edf_d, FP64 meshes): distance bit-exact to the FP64 argmin, index identical;
+  FP32 plain: |d - d_oracle| <= 1e-6 * bbox diagonal; index may differ only on near-ties;
+  FP32 + FP64 fixup: closest-triangle index identical to the FP64 argmin (lowest index on exact
+  ties), |d - d_oracle| <= 1e-6 * bbox diagonal, sign identical outside the 1e-6 * bbox band.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ssdf_amd as S
+from ssdf_amd import meshgen as mg
+from tests.test_oracle import brute_force_case, single_triangle_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _diag(m):
+    lo, hi = m.bbox()
+    return float(np.linalg.norm(hi - lo))
+
+
+def _dev(*arrs):
+    return [S.DeviceArray.from_host(a) for a in arrs]
+
+
+def _sdf_grid(m, n, lo, hi, flags, dtype=np.float32):
+    rm = S.ResidentMesh.from_mesh(m, with_normals=True, dtype=dtype)
+    N = n[0] * n[1] * n[2]
+    out = S.DeviceArray(N, dtype)
+    tri = S.DeviceArray(N, np.int32)
+    rm.query_grid(S.MODE_SDF, n, lo, hi, out, tri, flags=flags)
+    q, err = rm.last_stats()
+    assert err == 0
+    return out.download(), tri.download(), rm, q
+
+
+# ----------------------------------------------------------------------------------------------
+# the reference's own tests, through the C ABI
+# ----------------------------------------------------------------------------------------------
+def test_reference_single_triangle(gpu_lib):
+    m, x, y, z = single_triangle_case()
+    out = np.full(5, 1111111, np.float32)
+    assert S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out) == 0
+    assert np.all(np.isfinite(out)) and np.all(out >= 0)
+    assert np.abs(out - np.array([0, 0, 0, 1, 0])).max() < 1e-5  # tests/ssdf_test.cpp:61
+
+
+def test_reference_brute_force_case(gpu_lib, oracle):
+    m, x, y, z = brute_force_case()
+    out = np.full(20, 1e9)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.abs(out - np.sqrt(d2)).max() <= 1e-7  # tests/ssdf_brute_force_test.cpp:84,91
+    assert np.array_equal(out, np.sqrt(d2))  # FP64 path: bit-exact
+
+
+# ----------------------------------------------------------------------------------------------
+# config 1: UV sphere 48x24, 64^3 grid on [-1,1]^3
+# ----------------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c1(oracle):
+    m = mg.uv_sphere()
+    x, y, z = mg.grid_points(64, 64, 64, (-1, -1, -1), (1, 1, 1))
+    s, tri, sg, rg, dot = oracle.sdf(x, y, z, m)
+    d2, _ = oracle.bvh_argmin(x, y, z, m)
+    return m, (x, y, z), s, tri, d2
+
+
+def test_c1_fp64_edf_bit_exact(c1):
+    m, (x, y, z), s, tri, d2 = c1
+    md = m.as_dtype(np.float64)
+    out = np.full(x.size, 1e9)
+    S.edf(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), md.s0, md.s1, md.s2, md.sx, md.sy, md.sz,
+          out)
+    assert np.array_equal(out, np.sqrt(d2))
+
+
+def test_c1_fp64_closest_point_index(c1):
+    m, (x, y, z), s, tri, d2 = c1
+    md = m.as_dtype(np.float64)
+    n = x.size
+    outd = np.full(n, np.inf)
+    cx, cy, cz = np.zeros(n), np.zeros(n), np.zeros(n)
+    ot = np.full(n, -7, np.int32)
+    S.closest_point_bvh(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), md.s0, md.s1, md.s2, md.sx,
+                        md.sy, md.sz, outd, cx, cy, cz, ot)
+    assert np.array_equal(ot, tri)
+    assert np.array_equal(outd, np.sqrt(d2))
+    # closest point lies at the returned distance
+    dd = np.sqrt((cx - x) ** 2 + (cy - y) ** 2 + (cz - z) ** 2)
+    assert np.abs(dd - outd).max() < 1e-12
+
+
+def test_c1_fp32_plain_edf(c1):
+    m, (x, y, z), s, tri, d2 = c1
+    out = np.full(x.size, np.finfo(np.float32).max, np.float32)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    assert np.abs(out - np.sqrt(d2)).max() <= 1e-6 * _diag(m)
+
+
+@pytest.mark.parametrize("flags", [S.FLAG_FIXUP, S.FLAG_FIXUP | S.FLAG_SIGN_FP64])
+def test_c1_sdf_fixup_grid(c1, flags):
+    m, (x, y, z), s, tri, d2 = c1
+    out, otri, rm, nq = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), flags)
+    assert np.array_equal(otri, tri), f"{(otri != tri).sum()} index mismatches"
+    eps = 1e-6 * _diag(m)
+    assert np.abs(np.abs(out) - np.sqrt(d2)).max() <= eps
+    band = np.sqrt(d2) > eps
+    assert np.sum((np.sign(out) != np.sign(s)) & band) == 0
+
+
+def test_c1_sdf_plain_grid_sign_and_distance(c1):
+    m, (x, y, z), s, tri, d2 = c1
+    out, otri, rm, nq = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), 0)
+    eps = 1e-6 * _diag(m)
+    assert np.abs(np.abs(out) - np.sqrt(d2)).max() <= eps
+    assert np.sum((np.sign(out) != np.sign(s)) & (np.sqrt(d2) > eps)) == 0
+    assert nq == 0
+
+
+def test_c1_points_equal_grid(c1):
+    m, (x, y, z), s, tri, d2 = c1
+    out_g, tri_g, rm, _ = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), S.FLAG_FIXUP)
+    out_p = np.empty(x.size, np.float32)
+    tri_p = np.empty(x.size, np.int32)
+    S.sdf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out_p, tri_p, flags=S.FLAG_FIXUP)
+    assert np.array_equal(out_p, out_g) and np.array_equal(tri_p, tri_g)
+
+
+def test_gpu_brute_equals_oracle(oracle):
+    m = mg.uv_sphere(0.5, 24, 12)
+    x, y, z = mg.grid_points(30, 30, 30, (-1, -1, -1), (1, 1, 1))
+    rm = S.ResidentMesh.from_mesh(m, with_normals=False)
+    dx, dy, dz = _dev(x, y, z)
+    d2 = S.DeviceArray(x.size, np.float64)
+    t = S.DeviceArray(x.size, np.int32)
+    rm.brute_points(dx, dy, dz, d2, t)
+    rd2, rtri, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.array_equal(d2.download(), rd2) and np.array_equal(t.download(), rtri)
+
+
+def test_lbvh_depth_matches_canonical_replay(oracle):
+    m = mg.torus(0.6, 0.2, 100, 40, sigma=0.005, seed=9)
+    x, y, z = mg.grid_points(8, 8, 8, (-1, -1, -1), (1, 1, 1))
+    _, _, md, _ = oracle.lbvh_counts(x, y, z, m)
+    rm = S.ResidentMesh.from_mesh(m, with_normals=True)
+    info = rm.info()
+    assert info["maxdepth"] == md
+    assert info["open_edges"] == 0 and info["flipped_edges"] == 0
+
+
+# ----------------------------------------------------------------------------------------------
+# noisy torus (config 2's mesh family) at test size, and the staircase sign fixture
+# ----------------------------------------------------------------------------------------------
+def test_noisy_torus_fixup_parity(oracle):
+    m = mg.torus(0.6, 0.2, 125, 50, sigma=0.005, seed=2)
+    n = (48, 48, 40)
+    lo, hi = (-1, -1, -0.5), (1, 1, 0.5)
+    x, y, z = mg.grid_points(*n, lo, hi)
+    s, tri, _, _, _ = oracle.sdf(x, y, z, m)
+    out, otri, rm, nq = _sdf_grid(m, n, lo, hi, S.FLAG_FIXUP)
+    assert np.array_equal(otri, tri), f"{(otri != tri).sum()} index mismatches"
+    eps = 1e-6 * _diag(m)
+    assert np.abs(np.abs(out) - np.abs(s)).max() <= eps
+    assert np.sum((np.sign(out) != np.sign(s)) & (np.abs(s) > eps)) == 0
+
+
+def test_staircase_sample_sdf(gpu_lib):
+    """tests/sample_sdf_bvh_test.cpp through ssdf_sample_sdf_f: zero wrong signs, |err| < 1e-5."""
+    m, faces = mg.staircase()
+    N, H = 4, 0.25
+    n = 5 * (N + 2) + 1
+    lo, hi = np.float32(-H), np.float32((N + 1) * H)
+    out = np.empty(n ** 3, np.float32)
+    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, n, n, n, lo, lo, lo, hi, hi, hi, out)
+    hh = (float(hi) - float(lo)) / (n - 1)
+    g = np.float64(lo) + np.arange(n) * hh
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    d2 = np.full(P.shape[0], np.inf)
+    for axis, sign, flo, fhi in faces:
+        c = np.clip(P, flo, fhi)
+        d2 = np.minimum(d2, ((P - c) ** 2).sum(1))
+    d = np.sqrt(d2)
+    off = d >= 1e-6
+    ci, cj, ck = (np.floor(P[:, k] / H).astype(int) for k in range(3))
+    inside = (ci >= 0) & (cj >= 0) & (ck >= 0) & (ci < N) & (cj < N) & (ck < N) & (ci <= cj) & (ck >= 1) & (ck <= 2)
+    assert np.sum(((out < 0) != inside) & off) == 0
+    assert np.abs(np.abs(out) - d)[off].max() < 1e-5
+
+
+# ----------------------------------------------------------------------------------------------
+# semantics and edge cases
+# ----------------------------------------------------------------------------------------------
+def test_empty_mesh_leaves_out_untouched(gpu_lib):
+    x = np.zeros(10, np.float32)
+    out = np.full(10, 3.5, np.float32)
+    e = np.zeros(0, np.int32)
+    S.edf(x, x, x, e, e, e, np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32), out)
+    assert np.all(out == 3.5)
+
+
+def test_min_accumulate_and_strict_improvement(gpu_lib, oracle):
+    m = mg.uv_sphere(0.5, 16, 8)
+    x, y, z = mg.grid_points(9, 9, 9, (-1, -1, -1), (1, 1, 1))
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    cap = np.float32(0.3)
+    out = np.full(x.size, cap, np.float32)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    exp = np.minimum(np.sqrt(d2), cap)
+    assert np.abs(out - exp).max() <= 1e-6
+    # closest point: outputs untouched where the caller's distance is already smaller
+    outd = np.full(x.size, cap, np.float32)
+    cx = np.full(x.size, 9, np.float32)
+    cy, cz = cx.copy(), cx.copy()
+    ot = np.full(x.size, -5, np.int32)
+    S.closest_point_bvh(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, outd, cx, cy, cz, ot, flags=S.FLAG_FIXUP)
+    far = np.sqrt(d2) > cap * 1.001
+    assert np.all(ot[far] == -5) and np.all(cx[far] == 9) and np.all(outd[far] == cap)
+    near = np.sqrt(d2) < cap * 0.999
+    assert np.array_equal(ot[near], tri[near])
+
+
+@pytest.mark.parametrize("npts", [1, 63, 64, 65, 1000])
+def test_ragged_point_counts(gpu_lib, oracle, npts):
+    m = mg.icosphere(6)
+    rng = np.random.default_rng(npts)
+    x, y, z = (rng.uniform(-1.5, 1.5, npts).astype(np.float32) for _ in range(3))
+    out = np.empty(npts, np.float32)
+    ot = np.empty(npts, np.int32)
+    S.sdf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out, ot, flags=S.FLAG_FIXUP)
+    s, tri, _, _, _ = oracle.sdf(x, y, z, m, use_bvh=False)
+    assert np.array_equal(ot, tri)
+    assert np.abs(out - s).max() <= 1e-6 * 3.5
+
+
+@pytest.mark.parametrize("n", [(1, 1, 1), (5, 3, 7), (17, 1, 9), (33, 34, 35)])
+def test_ragged_grids(gpu_lib, oracle, n):
+    m = mg.uv_sphere(0.5, 20, 10)
+    lo, hi = (-0.9, -0.8, -0.7), (0.9, 0.8, 0.7)
+    out = np.empty(n[0] * n[1] * n[2], np.float32)
+    ot = np.empty_like(out, dtype=np.int32)
+    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, *n, *lo, *hi, out, ot, flags=S.FLAG_FIXUP)
+    x, y, z = mg.grid_points(*n, lo, hi)
+    s, tri, _, _, _ = oracle.sdf(x, y, z, m, use_bvh=False)
+    assert np.array_equal(ot, tri)
+    assert np.abs(out - s).max() <= 1e-6 * 2
+
+
+def test_single_and_degenerate_triangles(gpu_lib, oracle):
+    # one triangle (the LBVH root is a leaf)
+    m, x, y, z = single_triangle_case()
+    out = np.full(5, 1e30, np.float32)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    assert np.abs(out - np.array([0, 0, 0, 1, 0])).max() < 1e-6
+    # a soup with zero-area triangles and many coincident centroids (equal Morton keys)
+    rng = np.random.default_rng(4)
+    v = rng.uniform(-1, 1, (300, 3)).astype(np.float32)
+    v = np.concatenate([v, np.repeat(v[:10], 3, axis=0), np.zeros((60, 3), np.float32)])
+    t = np.arange(v.shape[0] // 3, dtype=np.int32)
+    soup = mg.Mesh(v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy(), 3 * t, 3 * t + 1, 3 * t + 2)
+    px, py, pz = (rng.uniform(-1.2, 1.2, 4000).astype(np.float32) for _ in range(3))
+    d2, tri, _, _ = oracle.brute_argmin(px, py, pz, soup)
+    sd = soup.as_dtype(np.float64)
+    out64 = np.full(4000, np.inf)
+    S.edf(px.astype(np.float64), py.astype(np.float64), pz.astype(np.float64), sd.s0, sd.s1, sd.s2, sd.sx, sd.sy,
+          sd.sz, out64)
+    assert np.array_equal(out64, np.sqrt(d2))
+    out32 = np.full(4000, np.inf, np.float32)
+    S.edf(px, py, pz, soup.s0, soup.s1, soup.s2, soup.sx, soup.sy, soup.sz, out32)
+    assert np.abs(out32 - np.sqrt(d2)).max() <= 1e-6 * 3.5
+
+
+def test_resident_mesh_repeatable(gpu_lib):
+    m = mg.torus(0.6, 0.2, 60, 24, sigma=0.004, seed=1)
+    rm = S.ResidentMesh.from_mesh(m)
+    n = (20, 20, 12)
+    outs = []
+    for _ in range(3):
+        o = S.DeviceArray(np.prod(n), np.float32)
+        t = S.DeviceArray(np.prod(n), np.int32)
+        rm.query_grid(S.MODE_SDF, n, (-1, -1, -0.4), (1, 1, 0.4), o, t, flags=S.FLAG_FIXUP)
+        outs.append((o.download(), t.download()))
+    for o, t in outs[1:]:
+        assert np.array_equal(o, outs[0][0]) and np.array_equal(t, outs[0][1])
+
+
+def test_errors_are_reported(gpu_lib):
+    m = mg.uv_sphere(0.5, 8, 4)
+    with pytest.raises(TypeError):
+        S.edf(np.zeros(2, np.float32), np.zeros(2, np.float32), np.zeros(2, np.float64), m.s0, m.s1, m.s2, m.sx, m.sy,
+              m.sz, np.zeros(2, np.float32))
+    rm = S.ResidentMesh.from_mesh(m, with_normals=False)
+    o = S.DeviceArray(8, np.float32)
+    with pytest.raises(S.SsdfError):
+        rm.query_grid(S.MODE_SDF, (2, 2, 2), (-1, -1, -1), (1, 1, 1), o)  # signed needs normals

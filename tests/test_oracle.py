@@ -1,0 +1,408 @@
+"""Pins the CPU oracle (oracle/) independently of its own formulas.
+
+What fixes each oracle routine:
+  pt_tri_d2 / pt_tri_cp   bit-exact against the reference's own point_triangle_barycentric
+                          (serial/edf/edf.hpp:252-387, built from /root/reference into oracle/_ref:
+                          same Ericson regions and operation order as serial/ssdf.hpp:50-121 for
+                          non-degenerate triangles), and against exact rational arithmetic by a
+                          different algorithm (plane projection + segment distances);
+  pt_box_d2 / sdist       the reference's known answers (tests/point_aabb_distance_test.cpp:24-35,
+                          tests/signed_distance_box_test.cpp:24-38) and the reference header itself;
+  edf / celllist          the reference's tests (tests/ssdf_test.cpp, tests/ssdf_brute_force_test.cpp
+                          with its std::mt19937(1) inputs regenerated) and exact brute force;
+  brute / bvh argmin      exact rational brute force on small inputs; bvh == brute bit for bit;
+  sign / pseudonormals    the staircase fixture of tests/sample_sdf_bvh_test.cpp (zero wrong signs,
+                          |err| < 1e-5) and analytic inside tests of the sphere and torus;
+  gwn                     closed forms: 1 inside, 0 outside a closed mesh; 1/2 on a face's plane
+                          far from its boundary is not used;
+  lbvh_counts             its FP32 distances agree with FP64 brute force; its depth with the GPU's.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from ssdf_amd import meshgen as mg
+from tests.mt19937 import MT19937, uniform
+
+
+# ----------------------------------------------------------------------------------------------
+# exact references
+# ----------------------------------------------------------------------------------------------
+def _sub(a, b):
+    return [a[i] - b[i] for i in range(3)]
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _cross(a, b):
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+
+def exact_seg_d2(p, a, b):
+    ab = _sub(b, a)
+    L2 = _dot(ab, ab)
+    s = Fraction(0) if L2 == 0 else min(max(_dot(_sub(p, a), ab) / L2, Fraction(0)), Fraction(1))
+    q = [a[i] + s * ab[i] for i in range(3)]
+    d = _sub(p, q)
+    return _dot(d, d)
+
+
+def exact_tri_d2(p, a, b, c):
+    """Exact squared distance by plane projection + inside test, else the nearest edge."""
+    p, a, b, c = ([Fraction(float(v)) for v in x] for x in (p, a, b, c))
+    n = _cross(_sub(b, a), _sub(c, a))
+    nn = _dot(n, n)
+    if nn != 0:
+        t = _dot(_sub(p, a), n) / nn
+        q = [p[i] - t * n[i] for i in range(3)]
+        u = _dot(_cross(_sub(b, q), _sub(c, q)), n)
+        v = _dot(_cross(_sub(c, q), _sub(a, q)), n)
+        w = _dot(_cross(_sub(a, q), _sub(b, q)), n)
+        if u >= 0 and v >= 0 and w >= 0:
+            return t * t * nn
+    return min(exact_seg_d2(p, a, b), exact_seg_d2(p, b, c), exact_seg_d2(p, c, a))
+
+
+def _random_cases(n, seed, dtype):
+    """Triangles in [-1,1]^3 and points around them, spread over all seven Voronoi regions."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(n):
+        a, b, c = rng.uniform(-1, 1, (3, 3))
+        w = rng.uniform(-0.8, 1.8, 3)
+        w /= w.sum() if abs(w.sum()) > 0.1 else 1.0
+        nrm = np.cross(b - a, c - a)
+        p = w[0] * a + w[1] * b + w[2] * c + rng.normal(0, 0.3) * nrm
+        cases.append(tuple(np.asarray(v, dtype=dtype) for v in (p, a, b, c)))
+    return cases
+
+
+def _regions_hit(oracle, cases):
+    # classify by the closest point's position to make sure all branches are exercised
+    hit = set()
+    for p, a, b, c in cases:
+        cp = oracle.pt_tri_cp(p, a, b, c)
+        if np.allclose(cp, a):
+            hit.add("A")
+        elif np.allclose(cp, b):
+            hit.add("B")
+        elif np.allclose(cp, c):
+            hit.add("C")
+        else:
+            on = [abs(np.linalg.norm(np.cross(q2 - q1, cp - q1))) < 1e-12 for q1, q2 in ((a, b), (a, c), (b, c))]
+            hit.add("AB" if on[0] else "AC" if on[1] else "BC" if on[2] else "F")
+    return hit
+
+
+# ----------------------------------------------------------------------------------------------
+# point-triangle distance
+# ----------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pt_tri_d2_matches_reference_header_bitwise(oracle, dtype):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference header shim not built (no /root/reference)")
+    import ctypes as C
+    cases = _random_cases(3000, 11, dtype)
+    assert _regions_hit(oracle, [tuple(np.float64(v) for v in cs) for cs in cases[:400]]) >= {
+        "A", "B", "C", "AB", "AC", "BC", "F"}
+    f = ref.ref_point_triangle_barycentric_d if dtype == np.float64 else ref.ref_point_triangle_barycentric_f
+    skipped = 0
+    for p, a, b, c in cases:
+        # edf.hpp:280-281 sends near-degenerate triangles to a segment fallback that ssdf.hpp:50-121
+        # does not have; compare only where both take the Voronoi-region path
+        ab, ac, bc = b - a, c - a, c - b
+        e2 = max(np.dot(ab, ab), np.dot(ac, ac), np.dot(bc, bc))
+        n = np.cross(ab, ac)
+        if np.dot(n, n) <= np.finfo(dtype).eps * dtype(64) * e2 * e2:
+            skipped += 1
+            continue
+        w = np.zeros(3, dtype)
+        args = [np.ascontiguousarray(v) for v in (p, a, b, c)]
+        r = f(*[C.c_void_p(v.ctypes.data) for v in args], C.c_void_p(w.ctypes.data))
+        o = oracle.pt_tri_d2(p, a, b, c, dtype=dtype)
+        assert np.asarray(o, dtype).tobytes() == np.asarray(r, dtype).tobytes(), (p, a, b, c, o, r)
+        # the reference's barycentric weights give the closest point the oracle returns
+        cp = oracle.pt_tri_cp(p, a, b, c, dtype=np.float64)
+        q = w[0].astype(np.float64) * a + w[1] * b + w[2] * c
+        assert np.linalg.norm(cp - q) <= (1e-12 if dtype == np.float64 else 2e-4)  # FP32 weights
+    assert skipped < len(cases) // 100
+
+
+def test_pt_tri_d2_against_exact_rationals(oracle):
+    cases = _random_cases(600, 5, np.float64)
+    for p, a, b, c in cases:
+        ex = exact_tri_d2(p, a, b, c)
+        o = oracle.pt_tri_d2(p, a, b, c)
+        L = max(1.0, float(np.abs(np.concatenate([p, a, b, c])).max()))
+        assert abs(math.sqrt(o) - math.sqrt(float(ex))) <= 16 * 2.0 ** -52 * L, (p, a, b, c)
+        of = oracle.pt_tri_d2(p, a, b, c, dtype=np.float32)
+        exf = exact_tri_d2(*(np.float32(v) for v in (p, a, b, c)))
+        assert abs(math.sqrt(of) - math.sqrt(float(exf))) <= 16 * 2.0 ** -23 * L
+
+
+def test_pt_tri_known_answers(oracle):
+    # tests/point_quad_closest_point_test.cpp:16-21: q=(0.25,0.25,1) over (0,0,0),(1,0,0),(0,1,0):
+    # weights (0.5, 0.25, 0.25), squared distance 1
+    p, a, b, c = np.array([0.25, 0.25, 1.0]), np.zeros(3), np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
+    assert oracle.pt_tri_d2(p, a, b, c) == 1.0
+    assert np.allclose(oracle.pt_tri_cp(p, a, b, c), [0.25, 0.25, 0.0], atol=1e-15)
+
+
+# ----------------------------------------------------------------------------------------------
+# box primitives: the reference's own known answers
+# ----------------------------------------------------------------------------------------------
+BOX_D2_CASES = [  # tests/point_aabb_distance_test.cpp:24-35, box [0,1]^3
+    ((0.5, 0.5, 0.5), 0.0), ((0.0, 0.0, 0.0), 0.0), ((1.0, 0.2, 0.8), 0.0), ((-1.0, 0.5, 0.5), 1.0),
+    ((0.5, 2.0, 0.5), 1.0), ((0.5, 0.5, -0.25), 0.0625), ((-1.0, -2.0, 3.0), 9.0)]
+SDIST_CASES = [  # tests/signed_distance_box_test.cpp:24-38, box [-1,1]x[-2,2]x[0.5,1.5]
+    ((0.0, 0.0, 1.0), -0.5), ((0.0, 0.0, 1.4), -0.1), ((0.9, 0.0, 1.0), -0.1), ((-0.8, 1.9, 1.0), -0.1),
+    ((1.0, 0.0, 1.0), 0.0), ((0.0, -2.0, 1.0), 0.0), ((0.0, 0.0, 1.5), 0.0), ((2.0, 0.0, 1.0), 1.0),
+    ((0.0, -3.0, 1.0), 1.0), ((0.0, 0.0, 3.0), 1.5), ((3.0, 4.0, -1.0), math.sqrt(8.0 + 2.25))]
+
+
+def test_box_known_answers(oracle):
+    for p, e in BOX_D2_CASES:
+        assert abs(oracle.pt_box_d2(p, (0, 0, 0), (1, 1, 1)) - e) <= 1e-12
+    for p, e in SDIST_CASES:
+        assert abs(oracle.pt_box_sdist(p, (-1, -2, 0.5), (1, 2, 1.5)) - e) <= 1e-12
+
+
+def test_box_primitives_match_reference_header(oracle):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference header shim not built (no /root/reference)")
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        p = rng.uniform(-2, 2, 3)
+        lo = rng.uniform(-1, 0.5, 3)
+        hi = lo + rng.uniform(0, 1, 3)
+        args = (p[0], p[1], p[2], lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+        assert oracle.pt_box_d2(p, lo, hi) == ref.ref_point_aabb_distance_squared_d(*args)
+        assert oracle.pt_box_sdist(p, lo, hi) == ref.ref_point_aabb_signed_distance_d(*args)
+        f = [np.float32(v) for v in args]
+        assert oracle.pt_box_d2(np.float32(p), np.float32(lo), np.float32(hi), np.float32) == \
+            ref.ref_point_aabb_distance_squared_f(*f)
+
+
+# ----------------------------------------------------------------------------------------------
+# the reference's engine tests, on the restated engines
+# ----------------------------------------------------------------------------------------------
+def single_triangle_case():
+    """tests/ssdf_test.cpp:12-32: one triangle, five points, out pre-filled with 1111111."""
+    m = mg.Mesh(np.array([0.0, 1.0, 0.5], np.float32), np.array([0.0, 0.0, 1.0], np.float32),
+                np.zeros(3, np.float32), np.array([0], np.int32), np.array([1], np.int32), np.array([2], np.int32))
+    x = np.array([0.5, 0.0, 1.0, 0.5, 0.25], np.float32)
+    y = np.array([0.5, 0.0, 0.0, 0.5, 0.25], np.float32)
+    z = np.array([0.0, 0.0, 0.0, 1.0, 0.0], np.float32)
+    return m, x, y, z
+
+
+def brute_force_case():
+    """tests/ssdf_brute_force_test.cpp:11-38: std::mt19937(1), U(-1,1); 50 triangles with their own
+    three vertices each, 20 points; out pre-filled with 1e9; FP64."""
+    rng = MT19937(1)
+    ntri, npt = 50, 20
+    sx, sy, sz = np.empty(3 * ntri), np.empty(3 * ntri), np.empty(3 * ntri)
+    for i in range(3 * ntri):
+        sx[i] = uniform(rng, -1.0, 1.0)
+        sy[i] = uniform(rng, -1.0, 1.0)
+        sz[i] = uniform(rng, -1.0, 1.0)
+    x, y, z = np.empty(npt), np.empty(npt), np.empty(npt)
+    for i in range(npt):
+        x[i] = uniform(rng, -1.0, 1.0)
+        y[i] = uniform(rng, -1.0, 1.0)
+        z[i] = uniform(rng, -1.0, 1.0)
+    t = np.arange(ntri, dtype=np.int32)
+    m = mg.Mesh(sx, sy, sz, 3 * t, 3 * t + 1, 3 * t + 2, "brute_force_case")
+    return m, x, y, z
+
+
+def test_mt19937_known_answers():
+    assert MT19937(5489)() == 3499211612  # the 10000th-output check of the C++ standard is for seed 5489
+    r = MT19937(5489)
+    for _ in range(9999):
+        r()
+    assert r() == 4123659995  # [rand.predef]: mt19937 default-seeded, 10000th invocation
+
+
+def test_reference_single_triangle(oracle):
+    m, x, y, z = single_triangle_case()
+    expected = np.array([0.0, 0.0, 0.0, 1.0, 0.0])  # closed form: 4 points on the triangle, 1 above
+    for fn in (oracle.edf, oracle.celllist):
+        out = np.full(5, 1111111, np.float32)
+        fn(x, y, z, m, out)
+        assert np.all(np.isfinite(out)) and np.all(out >= 0)
+        assert np.abs(out - expected).max() < 1e-5
+
+
+def test_reference_brute_force_case(oracle):
+    m, x, y, z = brute_force_case()
+    exact = np.array([math.sqrt(float(min(exact_tri_d2((x[p], y[p], z[p]), (m.sx[3 * t], m.sy[3 * t], m.sz[3 * t]),
+                                                       (m.sx[3 * t + 1], m.sy[3 * t + 1], m.sz[3 * t + 1]),
+                                                       (m.sx[3 * t + 2], m.sy[3 * t + 2], m.sz[3 * t + 2]))
+                                          for t in range(50)))) for p in range(20)])
+    for fn in (oracle.edf, oracle.celllist):
+        out = np.full(20, 1e9)
+        fn(x, y, z, m, out)
+        assert np.abs(out - exact).max() <= 1e-7  # the reference test's tolerance (84, 91)
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.abs(np.sqrt(d2) - exact).max() <= 1e-15
+
+
+def test_engines_min_accumulate_and_empty(oracle):
+    m, x, y, z = single_triangle_case()
+    out = np.full(5, 0.25, np.float32)  # smaller than some distances: kept
+    oracle.edf(x, y, z, m, out)
+    assert out[3] == np.float32(0.25) and out[0] == 0
+    empty = mg.Mesh(m.sx, m.sy, m.sz, m.s0[:0], m.s1[:0], m.s2[:0])
+    out = np.full(5, 7.0, np.float32)
+    oracle.edf(x, y, z, empty, out)
+    oracle.celllist(x, y, z, empty, out)
+    assert np.all(out == 7.0)
+
+
+# ----------------------------------------------------------------------------------------------
+# argmin ground truth
+# ----------------------------------------------------------------------------------------------
+def test_brute_argmin_exact_small(oracle):
+    m, x, y, z = brute_force_case()
+    d2, tri, ntie, _ = oracle.brute_argmin(x, y, z, m)
+    for p in range(20):
+        ex = [exact_tri_d2((x[p], y[p], z[p]), (m.sx[3 * t], m.sy[3 * t], m.sz[3 * t]),
+                           (m.sx[3 * t + 1], m.sy[3 * t + 1], m.sz[3 * t + 1]),
+                           (m.sx[3 * t + 2], m.sy[3 * t + 2], m.sz[3 * t + 2])) for t in range(50)]
+        assert tri[p] == int(np.argmin([float(e) for e in ex]))
+
+
+@pytest.mark.parametrize("meshname", ["sphere", "torus_small", "soup"])
+def test_bvh_argmin_equals_brute(oracle, meshname):
+    if meshname == "sphere":
+        m = mg.uv_sphere()
+    elif meshname == "torus_small":
+        m = mg.torus(0.6, 0.2, 60, 24, sigma=0.005, seed=7)
+    else:
+        rng = np.random.default_rng(1)
+        v = rng.uniform(-1, 1, (900, 3)).astype(np.float32)
+        t = np.arange(300, dtype=np.int32)
+        m = mg.Mesh(v[:, 0].copy(), v[:, 1].copy(), v[:, 2].copy(), 3 * t, 3 * t + 1, 3 * t + 2)
+    x, y, z = mg.grid_points(20, 20, 20, (-1, -1, -1), (1, 1, 1))
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    d2b, trib = oracle.bvh_argmin(x, y, z, m)
+    assert np.array_equal(d2, d2b) and np.array_equal(tri, trib)
+
+
+# ----------------------------------------------------------------------------------------------
+# sign
+# ----------------------------------------------------------------------------------------------
+def test_staircase_sign_fixture(oracle):
+    """tests/sample_sdf_bvh_test.cpp:92-132 on the oracle: 31^3 grid over [-H, 5H], exact distance to
+    the boundary faces, zero wrong signs off the surface, worst |err| < 1e-5."""
+    m, faces = mg.staircase()
+    N, H = 4, 0.25
+    n = 5 * (N + 2) + 1
+    lo, hi = np.float32(-H), np.float32((N + 1) * H)
+    x, y, z = mg.grid_points(n, n, n, (lo,) * 3, (hi,) * 3, np.float32)
+    s, tri, sg, rg, dot = oracle.sdf(x, y, z, m, use_bvh=False)
+    P = np.stack([x, y, z], 1).astype(np.float64)
+    d2 = np.full(P.shape[0], np.inf)
+    for axis, sign, flo, fhi in faces:
+        c = np.clip(P, flo, fhi)
+        d2 = np.minimum(d2, ((P - c) ** 2).sum(1))
+    d = np.sqrt(d2)
+    off = d >= 1e-6
+    ci, cj, ck = (np.floor(P[:, k] / H).astype(int) for k in range(3))
+    inside = (ci >= 0) & (cj >= 0) & (ck >= 0) & (ci < N) & (cj < N) & (ck < N) & (ci <= cj) & (ck >= 1) & (ck <= 2)
+    assert np.sum(((s < 0) != inside) & off) == 0
+    assert np.abs(np.abs(s) - d)[off].max() < 1e-5
+    assert oracle.pseudonormals(m)[3] == 0
+
+
+@pytest.mark.parametrize("shape", ["sphere", "torus"])
+def test_sign_closed_forms(oracle, shape):
+    if shape == "sphere":
+        m = mg.uv_sphere()
+        x, y, z = mg.grid_points(40, 40, 40, (-1, -1, -1), (1, 1, 1))
+        r = np.sqrt(x.astype(np.float64) ** 2 + y ** 2 + z ** 2)
+        inside = r < 0.5
+        sag = 0.5 * (1 - math.cos(math.pi / 24))  # chord sagitta of the coarser direction
+        far = np.abs(r - 0.5) > 2 * sag
+    else:
+        m = mg.torus(0.6, 0.2, 120, 48)
+        x, y, z = mg.grid_points(40, 40, 40, (-1, -1, -0.4), (1, 1, 0.4))
+        rho = np.sqrt(x.astype(np.float64) ** 2 + y ** 2)
+        t = np.sqrt((rho - 0.6) ** 2 + z.astype(np.float64) ** 2)
+        inside = t < 0.2
+        sag = 0.8 * (1 - math.cos(math.pi / 48)) + 0.8 * (1 - math.cos(math.pi / 120))
+        far = np.abs(t - 0.2) > 2 * sag
+    s, tri, sg, rg, dot = oracle.sdf(x, y, z, m)
+    assert np.sum(((sg < 0) != inside) & far) == 0
+    assert far.sum() > 0.8 * far.size
+
+
+def test_pseudonormals_sphere(oracle):
+    m = mg.uv_sphere()
+    fn, en, vn, op = oracle.pseudonormals(m)
+    assert op == 0
+    V = np.stack([m.sx, m.sy, m.sz], 1).astype(np.float64)
+    vnn = vn.reshape(-1, 3)
+    cosang = (vnn * V).sum(1) / np.linalg.norm(vnn, axis=1) / np.linalg.norm(V, axis=1)
+    assert cosang.min() > 0.999  # angle-weighted normals of a sphere are radial
+    assert np.allclose(np.linalg.norm(fn.reshape(-1, 3), axis=1), 1.0)
+    # an open mesh: delete one triangle -> its three edges become boundary edges
+    keep = np.ones(m.ntris, bool)
+    keep[100] = False
+    mo = mg.Mesh(m.sx, m.sy, m.sz, m.s0[keep].copy(), m.s1[keep].copy(), m.s2[keep].copy())
+    assert oracle.pseudonormals(mo)[3] == 3
+
+
+def test_gwn_closed_forms(oracle):
+    m = mg.uv_sphere()
+    x, y, z = mg.grid_points(12, 12, 12, (-1, -1, -1), (1, 1, 1))
+    w = oracle.gwn(x, y, z, m)
+    r = np.sqrt(x.astype(np.float64) ** 2 + y ** 2 + z ** 2)
+    far = np.abs(r - 0.5) > 0.02
+    assert np.abs(w - (r < 0.5))[far].max() < 1e-9
+
+
+# ----------------------------------------------------------------------------------------------
+# generators and the canonical replay
+# ----------------------------------------------------------------------------------------------
+def _orientation_ok(m):
+    e = np.concatenate([np.stack([m.s0, m.s1], 1), np.stack([m.s1, m.s2], 1), np.stack([m.s2, m.s0], 1)])
+    key = e[:, 0].astype(np.int64) * m.nverts + e[:, 1]
+    rkey = e[:, 1].astype(np.int64) * m.nverts + e[:, 0]
+    return len(np.unique(key)) == len(key) and np.array_equal(np.sort(key), np.sort(rkey))
+
+
+@pytest.mark.parametrize("name", ["uv", "torus", "ico", "dispico", "stair"])
+def test_generators_closed_oriented(name):
+    m = {"uv": lambda: mg.uv_sphere(), "torus": lambda: mg.torus(0.6, 0.2, 50, 20, 0.005, 3),
+         "ico": lambda: mg.icosphere(5), "dispico": lambda: mg.displaced_icosphere(2000, 4),
+         "stair": lambda: mg.staircase()[0]}[name]()
+    assert _orientation_ok(m)  # every directed edge appears once and its reverse once
+    if name == "uv":
+        assert (m.nverts, m.ntris) == (1106, 2208)
+
+
+def test_config_meshes_sizes():
+    t = mg.torus(0.6, 0.2, 500, 200, sigma=0.005, seed=2)
+    assert (t.nverts, t.ntris) == (100000, 200000)
+    assert mg.icosphere(10).ntris == 2000
+    ax = mg.grid_axis(256, -1.0, 1.0)
+    assert ax.dtype == np.float32 and ax[0] == -1.0 and ax[-1] == np.float32(1.0)
+    assert ax[1] == np.float32(-1.0) + np.float32(1) * (np.float32(2.0) / np.float32(255))
+
+
+def test_lbvh_replay_distances(oracle):
+    m = mg.torus(0.6, 0.2, 80, 30, sigma=0.005, seed=5)
+    x, y, z = mg.grid_points(16, 16, 16, (-1, -1, -1), (1, 1, 1))
+    nb, nt, md, d2f = oracle.lbvh_counts(x, y, z, m)
+    d2, _, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.abs(np.sqrt(d2f.astype(np.float64)) - np.sqrt(d2)).max() < 1e-6
+    assert 0 < md < 64
+    assert nb.min() >= 2 and nt.min() >= 1
