@@ -242,7 +242,7 @@ static int orc_lbvh_maxdepth(const orc_lbvh *b) {
     if (b->n < 2) return 0;
     int *stack = (int *)malloc(sizeof(int) * 2 * (b->n + 1));
     int sp = 0, maxd = 0;
-    stack[sp++] = 0, stack[sp++] = 1;
+    stack[sp++] = 0, stack[sp++] = 0; /* depth = edges from the root */
     while (sp) {
         const int depth = stack[--sp], node = stack[--sp];
         const int c[2] = {b->left[node], b->right[node]};

@@ -77,6 +77,24 @@ def build_oracle(with_reference: bool | None = None) -> None:
         subprocess.run(["make", "-s", "-C", str(oracle), "ref"], check=True)
 
 
+def build_cpp_tests() -> list[Path]:
+    """Compiles the C++ drop-in test programs (tests/cpp/*.cpp) against include/ and the library."""
+    root = PKG.parent
+    src_dir = root / "tests" / "cpp"
+    out_dir = src_dir / "_build"
+    out_dir.mkdir(exist_ok=True)
+    outs = []
+    for src in sorted(src_dir.glob("*.cpp")):
+        exe = out_dir / src.stem
+        cmd = ["g++", "-std=c++17", "-O1", "-I", str(root / "include"), str(src), "-L", str(LIBDIR), "-lssdf_amd",
+               "-Wl,-rpath," + str(LIBDIR), "-Wl,-rpath,$ORIGIN/../../../ssdf_amd/_lib", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src.name}:\n{r.stderr}")
+        outs.append(exe)
+    return outs
+
+
 def clean() -> None:
     shutil.rmtree(OBJDIR, ignore_errors=True)
 

@@ -47,3 +47,10 @@ def test_python_mirror_signatures_cover_header():
     import ssdf_amd
     missing = [n for n in declared_functions() if n not in ssdf_amd._SIGS]
     assert not missing, missing
+
+
+def test_cpp_dropin_programs_build():
+    """The C++ header compiles as a reference caller uses it, and links against the library."""
+    from ssdf_amd import build as B
+    exes = B.build_cpp_tests()
+    assert exes and all(e.exists() for e in exes)
