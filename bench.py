@@ -157,7 +157,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (queries shard by "
-              f"replicating the mesh, see DESIGN.md)", file=sys.stderr)
+              f"replicating the mesh and splitting the queries)", file=sys.stderr)
         sys.exit(2)
     m, n, lo, hi = make_workload(args)
     N = n[0] * n[1] * n[2]

@@ -2,14 +2,15 @@
 //
 // Build (one stream, no host round trip, graph-capturable):
 //   k_mesh_init   meta to neutral values
-//   k_tri_boxes   per-triangle FP32 AABB (outward-rounded for FP64 vertices) + centre/vertex bounds
+//   k_tri_boxes   per-triangle FP32 AABB (outward-rounded for FP64 vertices) + per-block partial
+//                 vertex/centre bounds, folded by k_reduce_bounds
 //   k_morton      63-bit Morton key of each box centre, normalised per axis by the centre bounds
 //   radix sort    (key, triangle) pairs, 8 passes of 8 bits, stable -> equal keys keep index order
 //   k_karras      internal node i's children and parent links (Karras, HPG 2012), ties in the keys
 //                 split by position, so the tree is a pure function of the mesh
-//   k_refit_pack  bottom-up: each leaf writes its box into the parent's child slot; the second
-//                 child to arrive at a node (atomic counter) writes the union one level up; the
-//                 leaf pass also packs the Morton-ordered 48-byte TriF records
+//   k_refit_pack  one thread per leaf: packs the Morton-ordered 48-byte TriF record and merges its
+//                 box into the ancestors' child slots with atomic min/max until a slot already
+//                 contains it; k_finalize_nodes turns the slots into the Nodes' float boxes
 //   k_depth       deepest leaf (sizes the traversal stack)
 // The same construction, run serially, is the oracle's canonical replay (oracle/oracle.c), so the
 // device tree and the replay's tree are identical.
@@ -27,27 +28,37 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *get_error() { return g_last_error.c_str(); }
 
+// Exclusive scan of m words in place, one block of 1024 threads walking coalesced 1024-word chunks:
+// wave scan by shuffles, then a scan of the 16 wave totals, then the running carry.
 __global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m) {
-    __shared__ uint32_t s[1024];
-    const unsigned tid = threadIdx.x;
-    const size_t per = (m + 1023) / 1024;
-    const size_t beg = (size_t)tid * per;
-    const size_t end = beg + per < m ? beg + per : m;
-    uint32_t sum = 0;
-    for (size_t i = beg; i < end; i++) sum += data[i];
-    s[tid] = sum;
-    __syncthreads();
-    for (unsigned off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = tid >= off ? s[tid - off] : 0u;
+    __shared__ uint32_t wsum[16];
+    const unsigned tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    uint32_t carry = 0;
+    for (size_t base = 0; base < m; base += 1024) {
+        const size_t i = base + tid;
+        const uint32_t v = i < m ? data[i] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
         __syncthreads();
-        s[tid] += v;
+        if (w == 0) {
+            uint32_t t = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t y = __shfl_up(t, o, 64);
+                if ((int)lane >= o) t += y;
+            }
+            if (lane < 16) wsum[lane] = t;
+        }
         __syncthreads();
-    }
-    uint32_t run = s[tid] - sum;
-    for (size_t i = beg; i < end; i++) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
+        const uint32_t wbase = w ? wsum[w - 1] : 0u;
+        if (i < m) data[i] = carry + wbase + x - v;
+        carry += wsum[15];
+        __syncthreads();
     }
 }
 
@@ -100,15 +111,19 @@ __device__ __forceinline__ float to_f_up<double>(double v) {
     return __double2float_ru(v);
 }
 
-// Triangle box (6 floats: min xyz, max xyz) + centre bounds and vertex bounds.
+// Triangle box (6 floats: min xyz, max xyz).  Each block also reduces its triangles' vertex
+// bounds and centre bounds to 12 partials (min xyz, max xyz of each); k_reduce_bounds folds the
+// partials into the mesh meta (two stages instead of contended atomics).
 template <typename G>
 __global__ __launch_bounds__(256) void k_tri_boxes(const int *__restrict__ s0, const int *__restrict__ s1,
                                                    const int *__restrict__ s2, const G *__restrict__ sx,
                                                    const G *__restrict__ sy, const G *__restrict__ sz, int n,
-                                                   float *__restrict__ boxes, MeshMeta *__restrict__ meta) {
+                                                   float *__restrict__ boxes, float *__restrict__ partials) {
+    __shared__ float red[4][12];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    float bmn[3] = {SSDF_INF, SSDF_INF, SSDF_INF}, bmx[3] = {-SSDF_INF, -SSDF_INF, -SSDF_INF};
-    float cmn[3] = {SSDF_INF, SSDF_INF, SSDF_INF}, cmx[3] = {-SSDF_INF, -SSDF_INF, -SSDF_INF};
+    float v[12];
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[d] = SSDF_INF, v[3 + d] = -SSDF_INF, v[6 + d] = SSDF_INF, v[9 + d] = -SSDF_INF;
     if (t < n) {
         const int iv[3] = {s0[t], s1[t], s2[t]};
         const G *co[3] = {sx, sy, sz};
@@ -122,21 +137,57 @@ __global__ __launch_bounds__(256) void k_tri_boxes(const int *__restrict__ s0, c
             const float flo = to_f_down<G>(lo), fhi = to_f_up<G>(hi);
             boxes[(size_t)6 * t + d] = flo;
             boxes[(size_t)6 * t + 3 + d] = fhi;
-            bmn[d] = flo, bmx[d] = fhi;
             const float cc = (flo + fhi) * 0.5f;
-            cmn[d] = cc, cmx[d] = cc;
+            v[d] = flo, v[3 + d] = fhi, v[6 + d] = cc, v[9 + d] = cc;
         }
     }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const float a = wave_min_f(bmn[d]), b = wave_max_f(bmx[d]);
-        const float c = wave_min_f(cmn[d]), e = wave_max_f(cmx[d]);
-        if ((threadIdx.x & 63) == 0 && a <= b) {
-            atomicMin(&meta->bbox_ord[d], float_to_ordered(a));
-            atomicMax(&meta->bbox_ord[3 + d], float_to_ordered(b));
-            atomicMin(&meta->cbox_ord[d], float_to_ordered(c));
-            atomicMax(&meta->cbox_ord[3 + d], float_to_ordered(e));
+        v[d] = wave_min_f(v[d]);
+        v[3 + d] = wave_max_f(v[3 + d]);
+        v[6 + d] = wave_min_f(v[6 + d]);
+        v[9 + d] = wave_max_f(v[9 + d]);
+    }
+    if (lane == 0)
+        for (int i = 0; i < 12; i++) red[w][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int i = threadIdx.x;
+        const bool is_min = (i % 6) < 3;
+        float r = red[0][i];
+        for (int ww = 1; ww < 4; ww++) r = is_min ? fminf(r, red[ww][i]) : fmaxf(r, red[ww][i]);
+        partials[(size_t)blockIdx.x * 12 + i] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_bounds(const float *__restrict__ partials, int nparts,
+                                                       MeshMeta *__restrict__ meta) {
+    __shared__ float red[4][12];
+    float v[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) v[i] = (i % 6) < 3 ? SSDF_INF : -SSDF_INF;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+            const float p = partials[(size_t)b * 12 + i];
+            v[i] = (i % 6) < 3 ? fminf(v[i], p) : fmaxf(v[i], p);
         }
+#pragma unroll
+    for (int i = 0; i < 12; i++) v[i] = (i % 6) < 3 ? wave_min_f(v[i]) : wave_max_f(v[i]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int i = 0; i < 12; i++) red[w][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int i = threadIdx.x;
+        const bool is_min = (i % 6) < 3;
+        float r = red[0][i];
+        for (int ww = 1; ww < 4; ww++) r = is_min ? fminf(r, red[ww][i]) : fmaxf(r, red[ww][i]);
+        if (i < 6)
+            meta->bbox_ord[i] = float_to_ordered(r);
+        else
+            meta->cbox_ord[i - 6] = float_to_ordered(r);
     }
 }
 
@@ -183,7 +234,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t *__restrict__ cod
                                                 int *__restrict__ parent, int *__restrict__ flags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
-    flags[i] = 0;
+    (void)flags;
     const int dir = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = kdelta(codes, n, i, i - dir);
     int lmax = 2;
@@ -226,14 +277,26 @@ __device__ __forceinline__ float to_f_near(G v) {
     return (float)v;
 }
 
+// Child-slot boxes during the build, as ordered ints (12 per internal node: left min xyz, max xyz,
+// right min xyz, max xyz), so that min/max merge with integer atomics.
+__global__ __launch_bounds__(256) void k_slots_init(int *__restrict__ slots, size_t nwords) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) slots[i] = ((i % 6) < 3) ? 0x7fffffff : (int)0x80000000;
+}
+
+// One thread per leaf (Morton position k): packs the TriF record, then climbs towards the root,
+// merging its box into each ancestor's child slot with atomic min/max, and stops at the first slot
+// that already contains it.  Stopping there is exact: each bound of that slot came from a leaf of
+// the same subtree, and that leaf carries the bound (or a tighter one) upward itself.  No fences
+// and no cross-XCD cache writebacks are needed; the next kernel sees every atomic's result.
 template <typename G>
 __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__ order, int n,
                                                     const int *__restrict__ s0, const int *__restrict__ s1,
                                                     const int *__restrict__ s2, const G *__restrict__ sx,
                                                     const G *__restrict__ sy, const G *__restrict__ sz,
                                                     const float *__restrict__ boxes, const int *__restrict__ parent,
-                                                    int *__restrict__ flags, Node *__restrict__ nodes,
-                                                    TriF *__restrict__ tris, MeshMeta *__restrict__ meta) {
+                                                    int *__restrict__ slots, TriF *__restrict__ tris,
+                                                    MeshMeta *__restrict__ meta) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int t = (int)order[k];
@@ -252,39 +315,48 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
         tr.pad1 = 0;
         tris[k] = tr;
     }
-    float mn[3], mx[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) mn[d] = boxes[(size_t)6 * t + d], mx[d] = boxes[(size_t)6 * t + 3 + d];
     if (n == 1) {
-        for (int d = 0; d < 3; d++) meta->root_box[d] = mn[d], meta->root_box[3 + d] = mx[d];
+        for (int d = 0; d < 6; d++) meta->root_box[d] = boxes[(size_t)6 * t + d];
         meta->root = ~0;
         return;
     }
+    int b[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) b[d] = float_to_ordered(boxes[(size_t)6 * t + d]);
     int p = parent[n - 1 + k];
     for (int guard = 0; guard < 4096; guard++) {
         const int pi = p >> 1, side = p & 1;
-        Node *nd = nodes + pi;
-        float *dmn = side ? nd->rmin : nd->lmin;
-        float *dmx = side ? nd->rmax : nd->lmax;
-#pragma unroll
-        for (int d = 0; d < 3; d++) dmn[d] = mn[d], dmx[d] = mx[d];
-        __threadfence();
-        const int old = atomicAdd(&flags[pi], 1);
-        if (old == 0) return;
-        __threadfence();
-        const volatile Node *vn = nd;
+        int *sl = slots + (size_t)12 * pi + 6 * side;
+        bool grew = false;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const float a = vn->lmin[d], b = vn->rmin[d];
-            const float c = vn->lmax[d], e = vn->rmax[d];
-            mn[d] = b < a ? b : a;
-            mx[d] = c < e ? e : c;
+            grew |= atomicMin(sl + d, b[d]) > b[d];
+            grew |= atomicMax(sl + 3 + d, b[3 + d]) < b[3 + d];
         }
-        if (pi == 0) {
-            for (int d = 0; d < 3; d++) meta->root_box[d] = mn[d], meta->root_box[3 + d] = mx[d];
-            return;
-        }
+        if (!grew || pi == 0) break;
         p = parent[pi];
+    }
+}
+
+// Ordered-int slots -> the float child boxes of each Node; the root's box into the meta.
+__global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ slots, int ninternal,
+                                                        Node *__restrict__ nodes, MeshMeta *__restrict__ meta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninternal) return;
+    const int *sl = slots + (size_t)12 * i;
+    Node *nd = nodes + i;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        nd->lmin[d] = ordered_to_float(sl[d]);
+        nd->lmax[d] = ordered_to_float(sl[3 + d]);
+        nd->rmin[d] = ordered_to_float(sl[6 + d]);
+        nd->rmax[d] = ordered_to_float(sl[9 + d]);
+    }
+    if (i == 0) {
+        for (int d = 0; d < 3; d++) {
+            meta->root_box[d] = fminf(ordered_to_float(sl[d]), ordered_to_float(sl[6 + d]));
+            meta->root_box[3 + d] = fmaxf(ordered_to_float(sl[3 + d]), ordered_to_float(sl[9 + d]));
+        }
     }
 }
 
@@ -314,54 +386,60 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
         set_error("build_lbvh: triangle count out of range");
         return SSDF_ERR_INVALID;
     }
+    const size_t nn = (size_t)n, ni = nn > 1 ? nn - 1 : 1;
     if (!m.meta) SSDF_HIP_TRY(hipMalloc(&m.meta, sizeof(MeshMeta)));
-    if (!m.nodes) SSDF_HIP_TRY(hipMalloc(&m.nodes, sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1)));
-    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriF) * (size_t)n));
-    m.device_bytes += sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1) + sizeof(TriF) * (size_t)n + sizeof(MeshMeta);
-    // workspace: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, flags 4 B per triangle
-    float *boxes = nullptr;
-    uint64_t *keys = nullptr;
-    uint32_t *vals = nullptr, *scratch = nullptr;
-    int *parent = nullptr, *flags = nullptr;
-    const size_t nn = (size_t)n;
-    SSDF_HIP_TRY(hipMalloc(&boxes, sizeof(float) * 6 * nn));
-    SSDF_HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * 2 * nn));
-    SSDF_HIP_TRY(hipMalloc(&vals, sizeof(uint32_t) * 2 * nn));
-    SSDF_HIP_TRY(hipMalloc(&parent, sizeof(int) * (2 * nn)));
-    SSDF_HIP_TRY(hipMalloc(&flags, sizeof(int) * (nn + 1)));
-    SSDF_HIP_TRY(hipMalloc(&scratch, sizeof(uint32_t) * radix_scratch_words(nn)));
+    if (!m.nodes) SSDF_HIP_TRY(hipMalloc(&m.nodes, sizeof(Node) * ni));
+    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriF) * nn));
+    m.device_bytes += sizeof(Node) * ni + sizeof(TriF) * nn + sizeof(MeshMeta);
+    // one workspace allocation: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, slots 48 B per
+    // triangle, block partials, radix scratch
+    const unsigned nb = nblk(nn, 256);
+    const size_t off_keys = 0, off_vals = off_keys + 16 * nn, off_boxes = off_vals + 8 * nn,
+                 off_parent = off_boxes + 24 * nn, off_slots = off_parent + 8 * nn, off_part = off_slots + 48 * ni,
+                 off_scr = off_part + 48 * (size_t)nb, total = off_scr + 4 * radix_scratch_words(nn);
+    char *ws = nullptr;
+    SSDF_HIP_TRY(hipMalloc(&ws, total));
+    uint64_t *keys = (uint64_t *)(ws + off_keys);
+    uint32_t *vals = (uint32_t *)(ws + off_vals);
+    float *boxes = (float *)(ws + off_boxes);
+    int *parent = (int *)(ws + off_parent);
+    int *slots = (int *)(ws + off_slots);
+    float *partials = (float *)(ws + off_part);
+    uint32_t *scratch = (uint32_t *)(ws + off_scr);
 
     hipLaunchKernelGGL(k_mesh_init, dim3(1), dim3(64), 0, stream, m.meta);
     if (m.dtype == 0)
-        hipLaunchKernelGGL(k_tri_boxes<float>, dim3(nblk(nn, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
-                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, n, boxes, m.meta);
+        hipLaunchKernelGGL(k_tri_boxes<float>, dim3(nb), dim3(256), 0, stream, m.s0, m.s1, m.s2, (const float *)m.sx,
+                           (const float *)m.sy, (const float *)m.sz, n, boxes, partials);
     else
-        hipLaunchKernelGGL(k_tri_boxes<double>, dim3(nblk(nn, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
-                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, n, boxes, m.meta);
-    hipLaunchKernelGGL(k_morton, dim3(nblk(nn, 256)), dim3(256), 0, stream, boxes, n, m.meta, keys, vals);
+        hipLaunchKernelGGL(k_tri_boxes<double>, dim3(nb), dim3(256), 0, stream, m.s0, m.s1, m.s2,
+                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, n, boxes, partials);
+    hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(256), 0, stream, partials, (int)nb, m.meta);
+    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(256), 0, stream, boxes, n, m.meta, keys, vals);
     uint64_t *skeys = nullptr;
     uint32_t *svals = nullptr;
     SSDF_HIP_TRY(radix_sort_pairs<uint64_t>(keys, keys + nn, vals, vals + nn, nn, 64, scratch, stream, &skeys, &svals));
-    if (n > 1)
-        hipLaunchKernelGGL(k_karras, dim3(nblk(nn - 1, 256)), dim3(256), 0, stream, skeys, n, m.nodes, parent, flags);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(nblk(nn - 1, 256)), dim3(256), 0, stream, skeys, n, m.nodes, parent,
+                           (int *)nullptr);
+        hipLaunchKernelGGL(k_slots_init, dim3(nblk(12 * ni, 256)), dim3(256), 0, stream, slots, 12 * ni);
+    }
     if (m.dtype == 0)
-        hipLaunchKernelGGL(k_refit_pack<float>, dim3(nblk(nn, 256)), dim3(256), 0, stream, svals, n, m.s0, m.s1,
-                           m.s2, (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, flags,
-                           m.nodes, m.tris, m.meta);
+        hipLaunchKernelGGL(k_refit_pack<float>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
+                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, slots, m.tris,
+                           m.meta);
     else
-        hipLaunchKernelGGL(k_refit_pack<double>, dim3(nblk(nn, 256)), dim3(256), 0, stream, svals, n, m.s0, m.s1,
-                           m.s2, (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent,
-                           flags, m.nodes, m.tris, m.meta);
-    hipLaunchKernelGGL(k_depth, dim3(nblk(nn, 256)), dim3(256), 0, stream, n, parent, m.meta);
+        hipLaunchKernelGGL(k_refit_pack<double>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
+                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent, slots,
+                           m.tris, m.meta);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_finalize_nodes, dim3(nblk(ni, 256)), dim3(256), 0, stream, slots, (int)ni, m.nodes,
+                           m.meta);
+        hipLaunchKernelGGL(k_depth, dim3(nb), dim3(256), 0, stream, n, parent, m.meta);
+    }
     SSDF_HIP_TRY(hipGetLastError());
-    // the frees below are stream-ordered only through the device-wide sync hipFree implies
     SSDF_HIP_TRY(hipStreamSynchronize(stream));
-    (void)hipFree(boxes);
-    (void)hipFree(keys);
-    (void)hipFree(vals);
-    (void)hipFree(parent);
-    (void)hipFree(flags);
-    (void)hipFree(scratch);
+    (void)hipFree(ws);
     return SSDF_OK;
 }
 

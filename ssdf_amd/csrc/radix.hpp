@@ -3,10 +3,10 @@
 // One pass per 8-bit digit, three kernels per pass:
 //   hist    : per-tile digit histogram in LDS, written digit-major (d * ntiles + tile);
 //   scan    : one-block exclusive scan of that digit-major table -> global scatter offsets;
-//   scatter : each wave ranks its 8 chunks of 64 keys in order — the lanes holding the same digit
+//   scatter : each wave ranks its 16 chunks of 64 keys in order — the lanes holding the same digit
 //             are found with 8 ballots (one per digit bit), a lane's rank is the popcount of its
 //             peers below it — so the pass is stable without any LDS sort.
-// Tiles are 2048 keys (256 threads x 8).  All launches go to the caller's stream and nothing is
+// Tiles are 4096 keys (256 threads x 16).  All launches go to the caller's stream and nothing is
 // read back, so a sort can sit inside a captured graph.
 #pragma once
 
@@ -15,7 +15,7 @@
 namespace ssdf_amd {
 
 constexpr int kRadixBlock = 256;
-constexpr int kRadixItems = 8;
+constexpr int kRadixItems = 16;
 constexpr int kRadixTile = kRadixBlock * kRadixItems;
 
 inline unsigned radix_tiles(size_t n) { return (unsigned)((n + kRadixTile - 1) / kRadixTile); }
@@ -39,8 +39,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *__restrict_
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m words in place, one block of 1024 threads: each thread sums a contiguous
-// segment, the block scans the 1024 partial sums in LDS, each thread rewrites its segment.
+// Exclusive scan of m words in place (one block of 1024 threads; lbvh.hip).
 __global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m);
 
 template <typename K>
