@@ -78,7 +78,10 @@ def run_gpu(args, m, n, lo, hi):
         L.ssdf_mesh_destroy(h)
 
     res = {}
-    for name, flags in (("fixup", S.FLAG_FIXUP), ("plain", 0)):
+    modes = (("fixup", S.FLAG_FIXUP), ("plain", 0))
+    if args.only:
+        modes = tuple(md for md in modes if md[0] == args.only)
+    for name, flags in modes:
         qt = []
         for _ in range(args.warmup):
             step(flags, qt)
@@ -92,6 +95,8 @@ def run_gpu(args, m, n, lo, hi):
         res[name] = {"ms_per_step": dt * 1e3, "query_ms": float(np.mean(qt)), "query_ms_min": float(np.min(qt)),
                      "mqps": N / dt / 1e6, "query_mqps": N / (np.mean(qt) * 1e-3) / 1e6}
         log(f"[bench] {name}: step {dt * 1e3:.3f} ms, query {np.mean(qt):.3f} ms")
+    if args.only:
+        return res
     # fixup statistics of one more fixup query on a resident mesh
     h = create()
     query(h, S.FLAG_FIXUP)
@@ -154,6 +159,10 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fixup", "plain"], default=None,
+                    help="time one mode only and print nothing else (for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="also run a plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (queries shard by "
@@ -163,6 +172,16 @@ def main():
     N = n[0] * n[1] * n[2]
     log(f"[bench] mesh {m.name}: {m.ntris} triangles; grid {n}; steps {args.steps} warmup {args.warmup}")
     g = run_gpu(args, m, n, lo, hi)
+    if args.copy_calibration:
+        import ssdf_amd as S
+        nbytes = args.copy_calibration << 20
+        src, dst = S.DeviceArray(nbytes // 4, np.float32), S.DeviceArray(nbytes // 4, np.float32)
+        S._check(S.lib().ssdf_device_copy(dst.ptr, src.ptr, nbytes, None), "ssdf_device_copy")
+        S._check(S.lib().ssdf_stream_synchronize(None), "sync")
+        log(f"[bench] copy calibration: {nbytes} bytes read + written by k_copy16")
+    if args.only:
+        log(json.dumps(g))
+        return
     line = {
         "metric": METRIC, "value": round(g["fixup"]["mqps"], 2), "unit": "Mqueries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(g["fixup"]["ms_per_step"], 4),

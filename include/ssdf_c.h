@@ -138,6 +138,8 @@ int ssdf_device_malloc(void **ptr, size_t bytes);
 int ssdf_device_free(void *ptr);
 int ssdf_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int ssdf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Streaming device-to-device copy by a plain kernel (16-byte aligned, multiple of 16 bytes). */
+int ssdf_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 int ssdf_stream_create(void **stream);
 int ssdf_stream_destroy(void *stream);
 int ssdf_stream_synchronize(void *stream);

@@ -70,6 +70,7 @@ _SIGS = {
     "ssdf_device_free": [_P],
     "ssdf_memcpy_h2d": [_P, _P, C.c_size_t],
     "ssdf_memcpy_d2h": [_P, _P, C.c_size_t],
+    "ssdf_device_copy": [_P, _P, C.c_size_t, _P],
     "ssdf_stream_create": [C.POINTER(_P)],
     "ssdf_stream_destroy": [_P],
     "ssdf_stream_synchronize": [_P],

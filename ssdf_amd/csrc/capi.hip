@@ -47,8 +47,8 @@ void ws_free(Workspace &w) {
     w = Workspace{};
 }
 
-// Exact FP64 argmin over every triangle (tiles of 256 TriF in LDS).
-__global__ __launch_bounds__(256) void k_brute_f64(const TriF *__restrict__ tris, int ntri,
+// Exact FP64 argmin over every triangle (tiles of 256 TriF headers in LDS).
+__global__ __launch_bounds__(256) void k_brute_f64(const TriP *__restrict__ tris, int ntri,
                                                    const float *__restrict__ x, const float *__restrict__ y,
                                                    const float *__restrict__ z, long long npoints,
                                                    double *__restrict__ out_d2, int *__restrict__ out_tri) {
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_brute_f64(const TriF *__restrict__ tris
     int bt = 0x7fffffff;
     for (int base = 0; base < ntri; base += 256) {
         __syncthreads();
-        if (base + (int)threadIdx.x < ntri) tile[threadIdx.x] = tris[base + threadIdx.x];
+        if (base + (int)threadIdx.x < ntri) tile[threadIdx.x] = tris[base + threadIdx.x].f;
         __syncthreads();
         const int m = std::min(256, ntri - base);
         for (int i = 0; i < m; i++) {
@@ -74,6 +74,13 @@ __global__ __launch_bounds__(256) void k_brute_f64(const TriF *__restrict__ tris
         out_d2[p] = best;
         out_tri[p] = bt == 0x7fffffff ? -1 : bt;
     }
+}
+
+// Plain streaming copy (16 B per lane per step): the reference byte count for calibrating the
+// HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 template <typename G>
@@ -482,6 +489,18 @@ int ssdf_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 }
 int ssdf_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     SSDF_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SSDF_OK;
+}
+int ssdf_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) {
+        set_error("ssdf_device_copy: 16-byte multiples and alignment");
+        return SSDF_ERR_INVALID;
+    }
+    const size_t n = bytes / 16;
+    if (n == 0) return SSDF_OK;
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, (hipStream_t)stream, (uint4 *)dst, (const uint4 *)src, n);
+    SSDF_HIP_TRY(hipGetLastError());
     return SSDF_OK;
 }
 int ssdf_stream_create(void **stream) {

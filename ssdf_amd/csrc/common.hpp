@@ -3,7 +3,8 @@
 // Layout in HBM (all per mesh, built once and kept resident):
 //   Node  (64 B, one per internal LBVH node): both children's AABBs + child references, read by a
 //         whole wave with one scalar 64-byte load during wave-coherent traversal.
-//   TriF  (48 B, one per triangle, in Morton/leaf order): FP32 vertices a, b, c + original index.
+//   TriP  (144 B, one per triangle, in Morton/leaf order): FP32 vertices a, b, c + original index
+//         (the 48-byte TriF header) + the precomputed constants of the FP32 distance routine.
 // Child references: r >= 0 is internal node r, r < 0 is leaf ~r (a position in the TriF array).
 #pragma once
 
@@ -29,9 +30,23 @@ static_assert(sizeof(Node) == 64, "Node must be 64 bytes");
 struct alignas(16) TriF {
     float a[3], b[3], c[3];
     int orig;
-    int pad0, pad1;
+    int flags;  // bit 0: degenerate (no usable normal)
+    int pad;
 };
 static_assert(sizeof(TriF) == 48, "TriF must be 48 bytes");
+
+// Triangle record of the FP32 traversal: the TriF header plus per-triangle constants of the
+// branch-free distance (tri_d2_fast), so that a wave reading it with scalar loads spends no VALU on
+// them: edges e0 = b-a, e1 = c-b, e2 = a-c; in-plane inward edge normals m_i = n x e_i; inverse
+// squared edge lengths; unit normal.  144 bytes, in leaf (Morton) order.
+struct alignas(16) TriP {
+    TriF f;
+    float e[9];
+    float m[9];
+    float inv[3];
+    float nh[3];
+};
+static_assert(sizeof(TriP) == 144, "TriP must be 144 bytes");
 
 // Error codes: SSDF_OK / SSDF_ERR_* from the public header.
 
@@ -154,6 +169,88 @@ __host__ __device__ __forceinline__ T pt_box_d2(const T px, const T py, const T 
     const T dy = (py < mny) ? (mny - py) : (py > mxy ? py - mxy : T(0));
     const T dz = (pz < mnz) ? (mnz - pz) : (pz > mxz ? pz - mxz : T(0));
     return dx * dx + dy * dy + dz * dz;
+}
+
+// FP32 point-AABB squared distance for pruning: per axis max(lo - p, p - hi, 0) (one v_max3),
+// sum with fused multiply-adds.  Same value as pt_box_d2 up to the rounding of the final sum.
+__host__ __device__ __forceinline__ float box_d2_fast(const float px, const float py, const float pz, const float *mn,
+                                             const float *mx) {
+    const float dx = fmaxf(fmaxf(mn[0] - px, px - mx[0]), 0.0f);
+    const float dy = fmaxf(fmaxf(mn[1] - py, py - mx[1]), 0.0f);
+    const float dz = fmaxf(fmaxf(mn[2] - pz, pz - mx[2]), 0.0f);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+// Branch-free FP32 squared point-triangle distance: the plane distance when the projection of p
+// falls inside (all three in-plane side tests non-negative), else the nearest of the three edge
+// segments.  Same quantity as the Voronoi-region routine pt_tri_d2 (serial/ssdf.hpp:50-121) to a
+// few ulps of the coordinates, with no per-lane branches or divisions (every lane of a wave runs the
+// same ~65 instructions); the exact FP64 routine re-decides near-ties.  Degenerate triangles carry
+// NaN side normals, so they take the segment branch.
+__host__ __device__ __forceinline__ float tri_d2_fast(const float px, const float py, const float pz, const TriP &t) {
+    const float pa0 = px - t.f.a[0], pa1 = py - t.f.a[1], pa2 = pz - t.f.a[2];
+    const float pb0 = px - t.f.b[0], pb1 = py - t.f.b[1], pb2 = pz - t.f.b[2];
+    const float pc0 = px - t.f.c[0], pc1 = py - t.f.c[1], pc2 = pz - t.f.c[2];
+    const float s0 = __builtin_fmaf(t.m[0], pa0, __builtin_fmaf(t.m[1], pa1, t.m[2] * pa2));
+    const float s1 = __builtin_fmaf(t.m[3], pb0, __builtin_fmaf(t.m[4], pb1, t.m[5] * pb2));
+    const float s2 = __builtin_fmaf(t.m[6], pc0, __builtin_fmaf(t.m[7], pc1, t.m[8] * pc2));
+    const float h = __builtin_fmaf(t.nh[0], pa0, __builtin_fmaf(t.nh[1], pa1, t.nh[2] * pa2));
+    // edge segments: t = clamp(e.p / |e|^2, 0, 1), r = p - t e
+    float u0 = __builtin_fmaf(t.e[0], pa0, __builtin_fmaf(t.e[1], pa1, t.e[2] * pa2)) * t.inv[0];
+    float u1 = __builtin_fmaf(t.e[3], pb0, __builtin_fmaf(t.e[4], pb1, t.e[5] * pb2)) * t.inv[1];
+    float u2 = __builtin_fmaf(t.e[6], pc0, __builtin_fmaf(t.e[7], pc1, t.e[8] * pc2)) * t.inv[2];
+    u0 = fminf(fmaxf(u0, 0.0f), 1.0f);
+    u1 = fminf(fmaxf(u1, 0.0f), 1.0f);
+    u2 = fminf(fmaxf(u2, 0.0f), 1.0f);
+    const float r00 = __builtin_fmaf(-u0, t.e[0], pa0), r01 = __builtin_fmaf(-u0, t.e[1], pa1),
+                r02 = __builtin_fmaf(-u0, t.e[2], pa2);
+    const float r10 = __builtin_fmaf(-u1, t.e[3], pb0), r11 = __builtin_fmaf(-u1, t.e[4], pb1),
+                r12 = __builtin_fmaf(-u1, t.e[5], pb2);
+    const float r20 = __builtin_fmaf(-u2, t.e[6], pc0), r21 = __builtin_fmaf(-u2, t.e[7], pc1),
+                r22 = __builtin_fmaf(-u2, t.e[8], pc2);
+    const float q0 = __builtin_fmaf(r00, r00, __builtin_fmaf(r01, r01, r02 * r02));
+    const float q1 = __builtin_fmaf(r10, r10, __builtin_fmaf(r11, r11, r12 * r12));
+    const float q2 = __builtin_fmaf(r20, r20, __builtin_fmaf(r21, r21, r22 * r22));
+    const bool inside = (s0 >= 0.0f) & (s1 >= 0.0f) & (s2 >= 0.0f);
+    return inside ? h * h : fminf(q0, fminf(q1, q2));
+}
+
+// Fills a TriP from FP32 vertices: the constants are evaluated in FP64 from the FP32 values and
+// rounded once.  A triangle whose normal vanishes against its edges (|n|^2 <= 1e-24 |e|^4) is
+// degenerate: NaN side normals send every query to the segment branch of tri_d2_fast.
+__host__ __device__ inline void make_trip(TriP &tr, const float *va, const float *vb, const float *vc, int orig) {
+    double v[3][3];
+    for (int d = 0; d < 3; d++) {
+        tr.f.a[d] = va[d], tr.f.b[d] = vb[d], tr.f.c[d] = vc[d];
+        v[0][d] = va[d], v[1][d] = vb[d], v[2][d] = vc[d];
+    }
+    double e[3][3], n[3];
+    for (int q = 0; q < 3; q++)
+        for (int d = 0; d < 3; d++) e[q][d] = v[(q + 1) % 3][d] - v[q][d];
+    // n = (b - a) x (c - a) = e0 x (-e2)
+    n[0] = e[0][2] * e[2][1] - e[0][1] * e[2][2];
+    n[1] = e[0][0] * e[2][2] - e[0][2] * e[2][0];
+    n[2] = e[0][1] * e[2][0] - e[0][0] * e[2][1];
+    const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    double emax = 0;
+    for (int q = 0; q < 3; q++) {
+        const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
+        emax = l2 > emax ? l2 : emax;
+    }
+    const bool degenerate = !(emax > 0) || !(nn > 1e-24 * emax * emax);
+    const double inv_nl = degenerate ? 0.0 : 1.0 / sqrt(nn);
+    for (int q = 0; q < 3; q++) {
+        const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
+        tr.inv[q] = l2 > 0 ? (float)(1.0 / l2) : 0.0f;
+        for (int d = 0; d < 3; d++) tr.e[3 * q + d] = (float)e[q][d];
+        const double m[3] = {n[1] * e[q][2] - n[2] * e[q][1], n[2] * e[q][0] - n[0] * e[q][2],
+                             n[0] * e[q][1] - n[1] * e[q][0]};
+        for (int d = 0; d < 3; d++) tr.m[3 * q + d] = degenerate ? __builtin_nanf("") : (float)(m[d] * inv_nl);
+    }
+    for (int d = 0; d < 3; d++) tr.nh[d] = (float)(n[d] * inv_nl);
+    tr.f.orig = orig;
+    tr.f.flags = degenerate ? 1 : 0;
+    tr.f.pad = 0;
 }
 
 // Monotone float <-> int mapping for atomicMin / atomicMax on floats.

@@ -8,7 +8,7 @@
 //   radix sort    (key, triangle) pairs, 8 passes of 8 bits, stable -> equal keys keep index order
 //   k_karras      internal node i's children and parent links (Karras, HPG 2012), ties in the keys
 //                 split by position, so the tree is a pure function of the mesh
-//   k_refit_pack  one thread per leaf: packs the Morton-ordered 48-byte TriF record and merges its
+//   k_refit_pack  one thread per leaf: packs the Morton-ordered 144-byte TriP record and merges its
 //                 box into the ancestors' child slots with atomic min/max until a slot already
 //                 contains it; k_finalize_nodes turns the slots into the Nodes' float boxes
 //   k_depth       deepest leaf (sizes the traversal stack)
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_slots_init(int *__restrict__ slots, siz
     if (i < nwords) slots[i] = ((i % 6) < 3) ? 0x7fffffff : (int)0x80000000;
 }
 
-// One thread per leaf (Morton position k): packs the TriF record, then climbs towards the root,
+// One thread per leaf (Morton position k): packs the TriP record, then climbs towards the root,
 // merging its box into each ancestor's child slot with atomic min/max, and stops at the first slot
 // that already contains it.  Stopping there is exact: each bound of that slot came from a leaf of
 // the same subtree, and that leaf carries the bound (or a tighter one) upward itself.  No fences
@@ -295,24 +295,19 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
                                                     const int *__restrict__ s2, const G *__restrict__ sx,
                                                     const G *__restrict__ sy, const G *__restrict__ sz,
                                                     const float *__restrict__ boxes, const int *__restrict__ parent,
-                                                    int *__restrict__ slots, TriF *__restrict__ tris,
+                                                    int *__restrict__ slots, TriP *__restrict__ tris,
                                                     MeshMeta *__restrict__ meta) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int t = (int)order[k];
     {
         const int iv[3] = {s0[t], s1[t], s2[t]};
-        TriF tr;
-#pragma unroll
-        for (int v = 0; v < 3; v++) {
-            float *dst = v == 0 ? tr.a : (v == 1 ? tr.b : tr.c);
-            dst[0] = to_f_near<G>(sx[iv[v]]);
-            dst[1] = to_f_near<G>(sy[iv[v]]);
-            dst[2] = to_f_near<G>(sz[iv[v]]);
-        }
-        tr.orig = t;
-        tr.pad0 = 0;
-        tr.pad1 = 0;
+        float va[3], vb[3], vc[3];
+        va[0] = to_f_near<G>(sx[iv[0]]), va[1] = to_f_near<G>(sy[iv[0]]), va[2] = to_f_near<G>(sz[iv[0]]);
+        vb[0] = to_f_near<G>(sx[iv[1]]), vb[1] = to_f_near<G>(sy[iv[1]]), vb[2] = to_f_near<G>(sz[iv[1]]);
+        vc[0] = to_f_near<G>(sx[iv[2]]), vc[1] = to_f_near<G>(sy[iv[2]]), vc[2] = to_f_near<G>(sz[iv[2]]);
+        TriP tr;
+        make_trip(tr, va, vb, vc, t);
         tris[k] = tr;
     }
     if (n == 1) {
@@ -389,8 +384,8 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
     const size_t nn = (size_t)n, ni = nn > 1 ? nn - 1 : 1;
     if (!m.meta) SSDF_HIP_TRY(hipMalloc(&m.meta, sizeof(MeshMeta)));
     if (!m.nodes) SSDF_HIP_TRY(hipMalloc(&m.nodes, sizeof(Node) * ni));
-    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriF) * nn));
-    m.device_bytes += sizeof(Node) * ni + sizeof(TriF) * nn + sizeof(MeshMeta);
+    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriP) * nn));
+    m.device_bytes += sizeof(Node) * ni + sizeof(TriP) * nn + sizeof(MeshMeta);
     // one workspace allocation: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, slots 48 B per
     // triangle, block partials, radix scratch
     const unsigned nb = nblk(nn, 256);

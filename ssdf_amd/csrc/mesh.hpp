@@ -28,7 +28,7 @@ struct DevMesh {
     bool owns_input = true;
     // LBVH
     Node *nodes = nullptr;  // nsel - 1 (at least 1)
-    TriF *tris = nullptr;   // nsel, leaf order
+    TriP *tris = nullptr;   // nsel, leaf order
     MeshMeta *meta = nullptr;  // device
     MeshMeta hmeta{};          // host copy after sync_meta()
     // pseudonormals (FP64): face 3T, edge 9T, vertex 3V

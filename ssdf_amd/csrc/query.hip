@@ -24,7 +24,7 @@
 namespace ssdf_amd {
 
 constexpr int kWaves = 4;
-constexpr int kCand = 8;
+constexpr int kCand = 12;
 constexpr int kNone = INT_MIN;
 
 struct alignas(16) StackEnt {
@@ -143,7 +143,7 @@ __device__ __forceinline__ void store_as(void *p, long long i, T v) {
 // Wave-coherent FP32 traversal (float queries, float meshes).
 // ------------------------------------------------------------------------------------------------
 template <int STACK, int QSRC, int OUT, bool FIX>
-__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriF *__restrict__ tris,
+__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ StackEnt stk_all[kWaves][STACK];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -188,10 +188,8 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
         }
         if (node >= 0) {
             const Node nd = nodes[node];
-            const float dl = pt_box_d2<float>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
-                                              nd.lmax[1], nd.lmax[2]);
-            const float dr = pt_box_d2<float>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
-                                              nd.rmax[1], nd.rmax[2]);
+            const float dl = box_d2_fast(qx, qy, qz, nd.lmin, nd.lmax);
+            const float dr = box_d2_fast(qx, qy, qz, nd.rmin, nd.rmax);
             const bool nl = dl <= thr, nr = dr <= thr;
             const uint64_t ml = __ballot(nl), mr = __ballot(nr);
             if (ml != 0 && mr != 0) {
@@ -227,9 +225,8 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
             }
         } else {
             const int k = ~node;
-            const TriF tr = tris[k];
-            const float d2 = pt_tri_d2<float>(qx, qy, qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2],
-                                              tr.c[0], tr.c[1], tr.c[2]);
+            const TriP tr = tris[k];
+            const float d2 = tri_d2_fast(qx, qy, qz, tr);
             if (!FIX) {
                 if (d2 <= thr) {
                     bd = d2;
@@ -263,7 +260,7 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
         while (sp > 0) {
             --sp;
             const StackEnt e = stk[sp];
-            const float dd = pt_box_d2<float>(qx, qy, qz, e.mn[0], e.mn[1], e.mn[2], e.mx[0], e.mx[1], e.mx[2]);
+            const float dd = box_d2_fast(qx, qy, qz, e.mn, e.mx);
             if (__ballot(dd <= thr) != 0) {
                 node = uniform_i(e.node);
                 break;
@@ -301,7 +298,7 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
 #pragma unroll
             for (int i = 0; i < kCand; i++) {
                 if (ck[i] >= 0 && cd[i] <= bt) {
-                    const TriF tr = tris[ck[i]];
+                    const TriF tr = tris[ck[i]].f;
                     const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
                                                        tr.b[2], tr.c[0], tr.c[1], tr.c[2]);
                     if (d < bestd || (d == bestd && tr.orig < bestt)) bestd = d, bestt = tr.orig, bestk = ck[i];
@@ -320,7 +317,7 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
         if (k_final >= 0) {
             const float o = fminf(out_init, dist);
             ((float *)a.out_d)[qid] = o;
-            if (a.out_tri) a.out_tri[qid] = tris[k_final].orig;
+            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
         }
         return;
     }
@@ -331,7 +328,7 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
         }
         return;
     }
-    const TriF tr = tris[k_final];
+    const TriF tr = tris[k_final].f;
     if (OUT == OUT_CLOSEST) {
         if (out_init > dist) {
             float cx, cy, cz;
@@ -464,7 +461,7 @@ __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
                     continue;
                 }
             } else {
-                const TriF tr = a.tris[~node];
+                const TriF tr = a.tris[~node].f;
                 double A[3], B[3], C[3];
                 tri_verts64<G>(a, tr, A, B, C);
                 const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
@@ -500,7 +497,7 @@ __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
             }
             continue;
         }
-        const TriF tr = a.tris[bk];
+        const TriF tr = a.tris[bk].f;
         double A[3], B[3], C[3];
         tri_verts64<G>(a, tr, A, B, C);
         if (OUT == OUT_CLOSEST) {

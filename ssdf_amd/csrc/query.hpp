@@ -23,7 +23,7 @@ enum : unsigned {
 struct QueryArgs {
     // mesh
     const Node *nodes;
-    const TriF *tris;
+    const TriP *tris;
     const MeshMeta *meta;
     const int *s0, *s1, *s2;
     const void *sx, *sy, *sz;  // caller-precision vertices (float or double)
