@@ -112,6 +112,13 @@ int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *
                      ptrdiff_t nspoints, const void *sx, const void *sy, const void *sz, int device_ptrs,
                      int with_normals, void *stream, ssdf_mesh_t *out);
 int ssdf_mesh_destroy(ssdf_mesh_t mesh);
+/* Rebuilds the LBVH (and, with_normals != 0, the pseudonormals) from the handle's current arrays,
+ * reusing its device memory: no allocation after the first build.  For meshes whose vertices move
+ * between queries (the reference's warm-start / FSI reuse workflow keeps the field, not the tree). */
+int ssdf_mesh_rebuild(ssdf_mesh_t mesh, int with_normals, void *stream);
+/* Copies new vertex coordinates (same count, host or device memory) into the handle and rebuilds. */
+int ssdf_mesh_update_vertices(ssdf_mesh_t mesh, const void *sx, const void *sy, const void *sz, int device_ptrs,
+                              int with_normals, void *stream);
 int ssdf_mesh_info(ssdf_mesh_t mesh, ssdf_mesh_info_t *info);
 
 /* Points (device SoA, mesh precision).  out_tri, out_cx/cy/cz may be NULL where the mode does not

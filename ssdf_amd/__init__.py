@@ -59,6 +59,8 @@ _SIGS = {
     "ssdf_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, C.c_uint],
     "ssdf_mesh_create": [C.c_int, _PD, _P, _P, _P, _PD, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
     "ssdf_mesh_destroy": [_P],
+    "ssdf_mesh_rebuild": [_P, C.c_int, _P],
+    "ssdf_mesh_update_vertices": [_P, _P, _P, _P, C.c_int, C.c_int, _P],
     "ssdf_mesh_info": [_P, C.POINTER(_MeshInfo)],
     "ssdf_mesh_query_points": [_P, C.c_int, _PD, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
     "ssdf_mesh_query_grid": [_P, C.c_int, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -337,6 +339,16 @@ class ResidentMesh:
         rc = lib().ssdf_mesh_brute_points(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_d2),
                                           _ptr(out_tri), C.c_void_p(stream.ptr) if stream else None)
         _check(rc, "ssdf_mesh_brute_points")
+
+    def rebuild(self, with_normals: bool = True, stream: Stream | None = None):
+        _check(lib().ssdf_mesh_rebuild(C.c_void_p(self.h), 1 if with_normals else 0,
+                                       C.c_void_p(stream.ptr) if stream else None), "ssdf_mesh_rebuild")
+
+    def update_vertices(self, sx, sy, sz, with_normals: bool = True, stream: Stream | None = None):
+        dev = isinstance(sx, DeviceArray)
+        _check(lib().ssdf_mesh_update_vertices(C.c_void_p(self.h), _ptr(sx), _ptr(sy), _ptr(sz), 1 if dev else 0,
+                                               1 if with_normals else 0, C.c_void_p(stream.ptr) if stream else None),
+               "ssdf_mesh_update_vertices")
 
     def last_stats(self):
         q, e = C.c_uint(), C.c_int()

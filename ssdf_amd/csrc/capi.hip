@@ -29,21 +29,21 @@ __global__ void k_reset_status(unsigned *status) {
     if (threadIdx.x < 2) status[threadIdx.x] = 0u;
 }
 
-int ws_reserve(Workspace &w, size_t n) {
-    if (!w.status) SSDF_HIP_TRY(hipMalloc(&w.status, 64));
+int ws_reserve(Workspace &w, size_t n, hipStream_t stream) {
+    if (!w.status) SSDF_HIP_TRY(dev_alloc((void **)&w.status, 64, stream));
     if (w.cap < n) {
-        if (w.list) SSDF_HIP_TRY(hipFree(w.list));
+        dev_free(w.list, stream);
         w.list = nullptr;
         w.cap = 0;
-        SSDF_HIP_TRY(hipMalloc(&w.list, sizeof(unsigned) * std::max<size_t>(n, 1)));
+        SSDF_HIP_TRY(dev_alloc((void **)&w.list, sizeof(unsigned) * std::max<size_t>(n, 1), stream));
         w.cap = n;
     }
     return SSDF_OK;
 }
 
-void ws_free(Workspace &w) {
-    (void)hipFree(w.status);
-    (void)hipFree(w.list);
+void ws_free(Workspace &w, hipStream_t stream) {
+    dev_free(w.status, stream);
+    dev_free(w.list, stream);
     w = Workspace{};
 }
 
@@ -87,6 +87,7 @@ template <typename G>
 int upload_mesh(DevMesh &m, ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp,
                 const G *sx, const G *sy, const G *sz, bool device_ptrs, hipStream_t stream) {
     m.dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    m.stream = stream;
     m.nsel = nsel;
     m.nsp = nsp;
     const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -94,13 +95,13 @@ int upload_mesh(DevMesh &m, ptrdiff_t nsel, const int *s0, const int *s1, const 
     int **idx[3] = {&m.s0, &m.s1, &m.s2};
     const int *isrc[3] = {s0, s1, s2};
     for (int k = 0; k < 3; k++) {
-        SSDF_HIP_TRY(hipMalloc(idx[k], std::max<size_t>(ib, 4)));
+        SSDF_HIP_TRY(dev_alloc((void **)idx[k], std::max<size_t>(ib, 4), stream));
         SSDF_HIP_TRY(hipMemcpyAsync(*idx[k], isrc[k], ib, kind, stream));
     }
     void **vtx[3] = {&m.sx, &m.sy, &m.sz};
     const G *vsrc[3] = {sx, sy, sz};
     for (int k = 0; k < 3; k++) {
-        SSDF_HIP_TRY(hipMalloc(vtx[k], std::max<size_t>(vb, 8)));
+        SSDF_HIP_TRY(dev_alloc(vtx[k], std::max<size_t>(vb, 8), stream));
         SSDF_HIP_TRY(hipMemcpyAsync(*vtx[k], vsrc[k], vb, kind, stream));
     }
     m.owns_input = true;
@@ -128,17 +129,29 @@ float mesh_diag(const DevMesh &m) {
 struct ssdf_mesh {
     DevMesh m;
     Workspace ws;
+    hipStream_t last_stream = nullptr;  // stream of the last query (its workspace use)
+    bool queried = false;
 };
 
 namespace {
 
 int prepare_args(ssdf_mesh *h, QueryArgs &a, size_t nq, unsigned flags, hipStream_t stream) {
-    if (int rc = ws_reserve(h->ws, (flags & QF_FIXUP) ? nq : 1)) return rc;
+    if (h->queried && h->last_stream != stream && h->ws.list) {
+        // the workspace was last used on another stream: order this use after it
+        hipEvent_t ev;
+        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SSDF_HIP_TRY(hipEventRecord(ev, h->last_stream));
+        SSDF_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
+        (void)hipEventDestroy(ev);
+    }
+    h->queried = true;
+    h->last_stream = stream;
+    if (int rc = ws_reserve(h->ws, (flags & QF_FIXUP) ? 2 * nq : 2, stream)) return rc;
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, h->ws.status);
     a.fix_count = h->ws.status;
     a.err_flag = (int *)(h->ws.status + 1);
     a.fix_list = h->ws.list;
-    a.fix_cap = (unsigned)std::min<size_t>(h->ws.cap, 0xffffffffu);
+    a.fix_cap = (unsigned)std::min<size_t>(h->ws.cap / 2, 0xffffffffu);
     const float L = mesh_scale(h->m);
     a.tau = kTau;
     a.band_abs = kBandRel * L;
@@ -175,8 +188,18 @@ int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const i
 
 void destroy_mesh(ssdf_mesh *h) {
     if (!h) return;
+    const hipStream_t st = h->m.stream;
+    if (h->queried && h->last_stream != st) {
+        // frees are ordered on the creation stream: make it wait for the last query's stream
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+            (void)hipEventRecord(ev, h->last_stream);
+            (void)hipStreamWaitEvent(st, ev, 0);
+            (void)hipEventDestroy(ev);
+        }
+    }
+    ws_free(h->ws, st);
     free_mesh(h->m);
-    ws_free(h->ws);
     delete h;
 }
 
@@ -402,6 +425,44 @@ int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *
     }
     return create_mesh(dtype, nselements, s0, s1, s2, nspoints, sx, sy, sz, device_ptrs != 0, with_normals != 0,
                        (hipStream_t)stream, out);
+}
+
+int ssdf_mesh_rebuild(ssdf_mesh_t mesh, int with_normals, void *stream) {
+    if (!mesh) {
+        set_error("ssdf_mesh_rebuild: null mesh");
+        return SSDF_ERR_INVALID;
+    }
+    if (mesh->m.nsel <= 0) return SSDF_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    if (st != mesh->m.stream) {
+        // the handle's buffers are ordered on its creation stream
+        hipEvent_t ev;
+        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SSDF_HIP_TRY(hipEventRecord(ev, mesh->m.stream));
+        SSDF_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
+        (void)hipEventDestroy(ev);
+        mesh->m.stream = st;
+    }
+    mesh->m.has_normals = false;
+    if (int rc = build_lbvh(mesh->m, st)) return rc;
+    if (with_normals)
+        if (int rc = build_normals(mesh->m, st)) return rc;
+    return sync_meta(mesh->m, st);
+}
+
+int ssdf_mesh_update_vertices(ssdf_mesh_t mesh, const void *sx, const void *sy, const void *sz, int device_ptrs,
+                              int with_normals, void *stream) {
+    if (!mesh || !sx || !sy || !sz) {
+        set_error("ssdf_mesh_update_vertices: null argument");
+        return SSDF_ERR_INVALID;
+    }
+    const size_t vb = (mesh->m.dtype ? sizeof(double) : sizeof(float)) * (size_t)mesh->m.nsp;
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipStream_t st = (hipStream_t)stream;
+    SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sx, sx, vb, kind, st));
+    SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sy, sy, vb, kind, st));
+    SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sz, sz, vb, kind, st));
+    return ssdf_mesh_rebuild(mesh, with_normals, stream);
 }
 
 int ssdf_mesh_destroy(ssdf_mesh_t mesh) {

@@ -212,7 +212,10 @@ __host__ __device__ __forceinline__ float tri_d2_fast(const float px, const floa
     const float q1 = __builtin_fmaf(r10, r10, __builtin_fmaf(r11, r11, r12 * r12));
     const float q2 = __builtin_fmaf(r20, r20, __builtin_fmaf(r21, r21, r22 * r22));
     const bool inside = (s0 >= 0.0f) & (s1 >= 0.0f) & (s2 >= 0.0f);
-    return inside ? h * h : fminf(q0, fminf(q1, q2));
+    // bit select, so that both branches stay straight-line code for the whole wave
+    const int msk = -(int)inside;
+    const int dp = __builtin_bit_cast(int, h * h), ds = __builtin_bit_cast(int, fminf(q0, fminf(q1, q2)));
+    return __builtin_bit_cast(float, (dp & msk) | (ds & ~msk));
 }
 
 // Fills a TriP from FP32 vertices: the constants are evaluated in FP64 from the FP32 values and

@@ -25,6 +25,35 @@
 namespace ssdf_amd {
 
 static thread_local std::string g_last_error;
+
+hipError_t dev_alloc(void **ptr, size_t bytes, hipStream_t stream) {
+    static std::once_flag once[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
+        std::call_once(once[dev], [dev] {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = ~0ull;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        });
+    }
+    return hipMallocAsync(ptr, bytes < 16 ? 16 : bytes, stream);
+}
+
+void dev_free(void *ptr, hipStream_t stream) {
+    if (ptr) (void)hipFreeAsync(ptr, stream);
+}
+
+int ensure_ws(DevMesh &m, size_t bytes, hipStream_t stream) {
+    if (m.ws_cap >= bytes) return SSDF_OK;
+    dev_free(m.ws, stream);
+    m.ws = nullptr;
+    m.ws_cap = 0;
+    SSDF_HIP_TRY(dev_alloc((void **)&m.ws, bytes, stream));
+    m.ws_cap = bytes;
+    return SSDF_OK;
+}
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *get_error() { return g_last_error.c_str(); }
 
@@ -382,18 +411,20 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
         return SSDF_ERR_INVALID;
     }
     const size_t nn = (size_t)n, ni = nn > 1 ? nn - 1 : 1;
-    if (!m.meta) SSDF_HIP_TRY(hipMalloc(&m.meta, sizeof(MeshMeta)));
-    if (!m.nodes) SSDF_HIP_TRY(hipMalloc(&m.nodes, sizeof(Node) * ni));
-    if (!m.tris) SSDF_HIP_TRY(hipMalloc(&m.tris, sizeof(TriP) * nn));
-    m.device_bytes += sizeof(Node) * ni + sizeof(TriP) * nn + sizeof(MeshMeta);
+    if (!m.nodes) {
+        SSDF_HIP_TRY(dev_alloc((void **)&m.meta, sizeof(MeshMeta), stream));
+        SSDF_HIP_TRY(dev_alloc((void **)&m.nodes, sizeof(Node) * ni, stream));
+        SSDF_HIP_TRY(dev_alloc((void **)&m.tris, sizeof(TriP) * nn, stream));
+        m.device_bytes += sizeof(Node) * ni + sizeof(TriP) * nn + sizeof(MeshMeta);
+    }
     // one workspace allocation: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, slots 48 B per
     // triangle, block partials, radix scratch
     const unsigned nb = nblk(nn, 256);
     const size_t off_keys = 0, off_vals = off_keys + 16 * nn, off_boxes = off_vals + 8 * nn,
                  off_parent = off_boxes + 24 * nn, off_slots = off_parent + 8 * nn, off_part = off_slots + 48 * ni,
                  off_scr = off_part + 48 * (size_t)nb, total = off_scr + 4 * radix_scratch_words(nn);
-    char *ws = nullptr;
-    SSDF_HIP_TRY(hipMalloc(&ws, total));
+    if (int rc = ensure_ws(m, total, stream)) return rc;
+    char *ws = m.ws;
     uint64_t *keys = (uint64_t *)(ws + off_keys);
     uint32_t *vals = (uint32_t *)(ws + off_vals);
     float *boxes = (float *)(ws + off_boxes);
@@ -433,8 +464,6 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
         hipLaunchKernelGGL(k_depth, dim3(nb), dim3(256), 0, stream, n, parent, m.meta);
     }
     SSDF_HIP_TRY(hipGetLastError());
-    SSDF_HIP_TRY(hipStreamSynchronize(stream));
-    (void)hipFree(ws);
     return SSDF_OK;
 }
 
@@ -558,21 +587,22 @@ __global__ void k_meta_zero_edges(MeshMeta *meta) {
 
 int build_normals(DevMesh &m, hipStream_t stream) {
     const size_t n = (size_t)m.nsel, V = (size_t)m.nsp;
-    if (m.has_normals) return SSDF_OK;
-    if (!m.face_n) SSDF_HIP_TRY(hipMalloc(&m.face_n, sizeof(double) * 3 * n));
-    if (!m.edge_n) SSDF_HIP_TRY(hipMalloc(&m.edge_n, sizeof(double) * 9 * n));
-    if (!m.vertex_n) SSDF_HIP_TRY(hipMalloc(&m.vertex_n, sizeof(double) * 3 * V));
-    m.device_bytes += sizeof(double) * (12 * n + 3 * V);
+    if (!m.face_n) {
+        SSDF_HIP_TRY(dev_alloc((void **)&m.face_n, sizeof(double) * 3 * n, stream));
+        SSDF_HIP_TRY(dev_alloc((void **)&m.edge_n, sizeof(double) * 9 * n, stream));
+        SSDF_HIP_TRY(dev_alloc((void **)&m.vertex_n, sizeof(double) * 3 * V, stream));
+        m.device_bytes += sizeof(double) * (12 * n + 3 * V);
+    }
     const size_t e = 3 * n;
-    double *angle = nullptr;
-    uint32_t *vkeys = nullptr, *vvals = nullptr, *evals = nullptr, *scratch = nullptr;
-    uint64_t *ekeys = nullptr;
-    SSDF_HIP_TRY(hipMalloc(&angle, sizeof(double) * e));
-    SSDF_HIP_TRY(hipMalloc(&vkeys, sizeof(uint32_t) * 2 * e));
-    SSDF_HIP_TRY(hipMalloc(&vvals, sizeof(uint32_t) * 2 * e));
-    SSDF_HIP_TRY(hipMalloc(&ekeys, sizeof(uint64_t) * 2 * e));
-    SSDF_HIP_TRY(hipMalloc(&evals, sizeof(uint32_t) * 2 * e));
-    SSDF_HIP_TRY(hipMalloc(&scratch, sizeof(uint32_t) * radix_scratch_words(e)));
+    // workspace: angle 8 B, vertex keys/vals 2 x 2 x 4 B, edge keys 2 x 8 B, edge vals 2 x 4 B per
+    // corner, radix scratch
+    const size_t off_angle = 0, off_vk = off_angle + 8 * e, off_vv = off_vk + 8 * e, off_ek = off_vv + 8 * e,
+                 off_ev = off_ek + 16 * e, off_scr = off_ev + 8 * e, total = off_scr + 4 * radix_scratch_words(e);
+    if (int rc = ensure_ws(m, total, stream)) return rc;
+    double *angle = (double *)(m.ws + off_angle);
+    uint32_t *vkeys = (uint32_t *)(m.ws + off_vk), *vvals = (uint32_t *)(m.ws + off_vv);
+    uint64_t *ekeys = (uint64_t *)(m.ws + off_ek);
+    uint32_t *evals = (uint32_t *)(m.ws + off_ev), *scratch = (uint32_t *)(m.ws + off_scr);
     hipLaunchKernelGGL(k_meta_zero_edges, dim3(1), dim3(1), 0, stream, m.meta);
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * V, 256)), dim3(256), 0, stream, m.vertex_n, 3 * V, 0.0);
     if (m.dtype == 0)
@@ -597,32 +627,27 @@ int build_normals(DevMesh &m, hipStream_t stream) {
     hipLaunchKernelGGL(k_edge_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, sek, sev, e, m.s0, m.s1, m.s2,
                        m.face_n, m.edge_n, m.meta);
     SSDF_HIP_TRY(hipGetLastError());
-    SSDF_HIP_TRY(hipStreamSynchronize(stream));
-    (void)hipFree(angle);
-    (void)hipFree(vkeys);
-    (void)hipFree(vvals);
-    (void)hipFree(ekeys);
-    (void)hipFree(evals);
-    (void)hipFree(scratch);
     m.has_normals = true;
     return SSDF_OK;
 }
 
 void free_mesh(DevMesh &m) {
+    hipStream_t st = m.stream;
     if (m.owns_input) {
-        (void)hipFree(m.s0);
-        (void)hipFree(m.s1);
-        (void)hipFree(m.s2);
-        (void)hipFree(m.sx);
-        (void)hipFree(m.sy);
-        (void)hipFree(m.sz);
+        dev_free(m.s0, st);
+        dev_free(m.s1, st);
+        dev_free(m.s2, st);
+        dev_free(m.sx, st);
+        dev_free(m.sy, st);
+        dev_free(m.sz, st);
     }
-    (void)hipFree(m.nodes);
-    (void)hipFree(m.tris);
-    (void)hipFree(m.meta);
-    (void)hipFree(m.face_n);
-    (void)hipFree(m.edge_n);
-    (void)hipFree(m.vertex_n);
+    dev_free(m.nodes, st);
+    dev_free(m.tris, st);
+    dev_free(m.meta, st);
+    dev_free(m.face_n, st);
+    dev_free(m.edge_n, st);
+    dev_free(m.vertex_n, st);
+    dev_free(m.ws, st);
     m = DevMesh{};
 }
 

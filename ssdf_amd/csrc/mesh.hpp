@@ -35,7 +35,21 @@ struct DevMesh {
     double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
     bool has_normals = false;
     size_t device_bytes = 0;
+    hipStream_t stream = nullptr;  // allocations and frees are ordered on this stream
+    // build workspace, kept across rebuilds (the LBVH build and the normals build run one after the
+    // other on `stream` and share it)
+    char *ws = nullptr;
+    size_t ws_cap = 0;
 };
+
+// Grows m.ws to at least `bytes` (stream-ordered); no-op when it is large enough.
+int ensure_ws(DevMesh &m, size_t bytes, hipStream_t stream);
+
+// Stream-ordered device memory from the device's default pool, which keeps freed blocks for reuse
+// (release threshold raised to the maximum at first use): repeated builds and queries allocate
+// without a device-wide synchronisation.
+hipError_t dev_alloc(void **ptr, size_t bytes, hipStream_t stream);
+void dev_free(void *ptr, hipStream_t stream);
 
 // Uploads nothing: builds the LBVH of a mesh whose SoA arrays are already on the device.
 int build_lbvh(DevMesh &m, hipStream_t stream);

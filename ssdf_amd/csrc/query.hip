@@ -9,10 +9,10 @@
 // wave evaluates it (point_triangle_dist_sq, serial/ssdf.hpp:50-121, in FP32).
 //
 // FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
-// FP32 error bound of a distance, and each lane keeps the kCand nearest triangles inside that band.
-// After traversal, a lane whose band holds one triangle is final; two or more are re-evaluated in
-// FP64 (same formula, contraction off -> the oracle's bits) and the lowest index wins exact ties;
-// a lane whose list overflowed is queued for k_query_exact (full FP64 traversal).
+// FP32 error bound of a distance, and each lane lists the triangles inside that band.  After
+// traversal, a lane whose band holds one triangle is final; two or more are re-evaluated in FP64
+// (same formula, contraction off -> the oracle's bits) and the lowest index wins exact ties; a lane
+// whose list overflowed is queued for k_query_exact (full FP64 traversal).
 //
 // k_query_exact: one lane per query, FP64 box and triangle tests, private stack; exact argmin with
 // the canonical tie-break.  Used for FP64 meshes and for the fixup queue.
@@ -24,15 +24,8 @@
 namespace ssdf_amd {
 
 constexpr int kWaves = 4;
-constexpr int kCand = 12;
+constexpr int kCand = 16;
 constexpr int kNone = INT_MIN;
-
-struct alignas(16) StackEnt {
-    float mn[3];
-    int node;
-    float mx[3];
-    int pad;
-};
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -141,13 +134,21 @@ __device__ __forceinline__ void store_as(void *p, long long i, T v) {
 
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).
+//
+// Stack: wave-uniform ints in LDS, each naming where the pending subtree hangs (parent << 1 | side);
+// a pop re-reads that child's box from the parent record with scalar loads and re-tests it.
+// FIX candidates: per lane, an append-only list of leaf positions in LDS ([kCand][256], lane-major,
+// conflict-free).  A triangle enters when its FP32 d2 is inside band(best); a new best whose own
+// band lies below the old best empties the list first (every old entry is >= the old best, so
+// none of them can be in the new band).  The list therefore always holds every triangle inside the
+// current band, plus possibly a few outside it, which the FP64 evaluation rejects.
 // ------------------------------------------------------------------------------------------------
 template <int STACK, int QSRC, int OUT, bool FIX>
 __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
-    __shared__ StackEnt stk_all[kWaves][STACK];
+    static_assert(STACK == 64 || STACK == 128, "the traversal stack lives in one or two VGPRs");
+    __shared__ int cand[FIX ? kCand : 1][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    StackEnt *stk = stk_all[w];
     const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
     const unsigned wave_id = blk * kWaves + (unsigned)w;
     long long qid = 0;
@@ -159,164 +160,133 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
     const float tau1 = 1.0f + a.tau;
     const float babs = fmaxf(a.band_abs, 0x1p-18f * fmaxf(fabsf(qx), fmaxf(fabsf(qy), fabsf(qz))));
 
-    // strict upper bound from the caller's current distance (EDF / CLOSEST), inclusive form
-    float init_thr = SSDF_INF;
+    // The caller's current distance bounds the search strictly (EDF / CLOSEST).  Plain mode keeps
+    // `best` and prunes with box < best; FIX keeps an inclusive bound thr >= best (its band).
     float out_init = SSDF_INF;
+    float init2 = SSDF_INF;
     if ((OUT == OUT_EDF || OUT == OUT_CLOSEST) && valid) {
         out_init = ((const float *)a.out_d)[qid];
-        init_thr = prev_f(out_init * out_init);
+        init2 = out_init * out_init;
     }
-    float thr = valid ? init_thr : -SSDF_INF;
+    float best = valid ? init2 : -SSDF_INF;  // plain: strict bound (invalid lanes never pass)
+    const float init_thr = __builtin_bit_cast(float, __builtin_bit_cast(int, init2) - 1);  // largest below init2
+    float thr = valid ? init_thr : -SSDF_INF;  // FIX: inclusive bound
+    int bk = -1, ncand = 0;
+    float bestd = SSDF_INF;  // FIX: FP32 best distance found
 
-    float bd = SSDF_INF;
-    int bk = -1;
-    float cd[kCand];
-    int ck[kCand];
-    float dropped = SSDF_INF;
-    if (FIX) {
-#pragma unroll
-        for (int i = 0; i < kCand; i++) cd[i] = SSDF_INF, ck[i] = -1;
-    }
-
-    int node = uniform_i(a.meta->root);
+    // Stack of pending subtrees: entry i sits in lane i % 64 of VGPR i / 64 (written by a lane
+    // select at the wave-uniform depth sp, read back with v_readlane), each naming where the subtree
+    // hangs: parent << 1 | side.
+    int stackv = 0, stackv_hi = 0;
     int sp = 0;
-    long long iters = 0;
+    bool overflow = false;
+    int iters = 0;
+    const int max_iters = (int)a.max_iters;
+    int node = uniform_i(a.meta->root);
     for (;;) {
-        if (++iters > a.max_iters) {
-            if (lane == 0) atomicOr(a.err_flag, 2);
-            break;
-        }
-        if (node >= 0) {
+        // descend through internal nodes, nearest child first by majority vote; the decision and the
+        // push are uniform selects, so a visit is straight-line code
+        while (node >= 0) {
             const Node nd = nodes[node];
             const float dl = box_d2_fast(qx, qy, qz, nd.lmin, nd.lmax);
             const float dr = box_d2_fast(qx, qy, qz, nd.rmin, nd.rmax);
-            const bool nl = dl <= thr, nr = dr <= thr;
-            const uint64_t ml = __ballot(nl), mr = __ballot(nr);
-            if (ml != 0 && mr != 0) {
-                const uint64_t votel = __ballot(nl && (!nr || dl <= dr));
-                const bool lfirst = 2 * __popcll(votel) >= __popcll(ml | mr);
-                const int nearc = lfirst ? nd.left : nd.right;
-                if (sp < STACK) {
-                    if (lane == 0) {
-                        StackEnt e;
-                        e.node = lfirst ? nd.right : nd.left;
-                        e.pad = 0;
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            e.mn[d] = lfirst ? nd.rmin[d] : nd.lmin[d];
-                            e.mx[d] = lfirst ? nd.rmax[d] : nd.lmax[d];
-                        }
-                        stk[sp] = e;
-                    }
-                    sp++;
-                } else if (lane == 0) {
-                    atomicOr(a.err_flag, 1);
-                }
-                node = uniform_i(nearc);
-                continue;
+            const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
+            const uint64_t mr = __ballot(FIX ? dr <= thr : dr < best);
+            const uint64_t mle = __ballot(dl <= dr);
+            const bool both = (ml != 0) & (mr != 0);
+            const bool lfirst = 2 * __popcll(ml & (~mr | mle)) >= __popcll(ml | mr);
+            const int ent = (node << 1) | (lfirst ? 1 : 0);
+            if (STACK == 64) {
+                stackv = (both && lane == sp) ? ent : stackv;
+            } else {
+                stackv = (both && lane == sp) ? ent : stackv;
+                stackv_hi = (both && lane == sp - 64) ? ent : stackv_hi;
             }
-            if (ml != 0) {
-                node = uniform_i(nd.left);
-                continue;
-            }
-            if (mr != 0) {
-                node = uniform_i(nd.right);
-                continue;
-            }
-        } else {
+            sp += both ? 1 : 0;
+            node = uniform_i(both ? (lfirst ? nd.left : nd.right)
+                                  : (ml != 0 ? nd.left : (mr != 0 ? nd.right : kNone)));
+        }
+        overflow |= sp > STACK;
+        if (node != kNone) {
             const int k = ~node;
             const TriP tr = tris[k];
             const float d2 = tri_d2_fast(qx, qy, qz, tr);
             if (!FIX) {
-                if (d2 <= thr) {
-                    bd = d2;
+                if (d2 < best) best = d2, bk = k;
+            } else if (d2 <= thr) {
+                if (d2 < bestd) {
+                    const float nb = band_sq(d2, tau1, babs);
+                    if (nb < bestd) ncand = 0;
+                    bestd = d2;
                     bk = k;
-                    thr = prev_f(d2);
+                    thr = fminf(nb, init_thr);
                 }
-            } else {
-                const bool ins = d2 <= thr;
-                if (__ballot(ins) != 0) {
-                    if (ins) {
-                        float xd = d2;
-                        int xk = k;
-#pragma unroll
-                        for (int i = 0; i < kCand; i++) {
-                            const bool sw = xd < cd[i];
-                            const float td = cd[i];
-                            const int tk = ck[i];
-                            cd[i] = sw ? xd : td;
-                            ck[i] = sw ? xk : tk;
-                            xd = sw ? td : xd;
-                            xk = sw ? tk : xk;
-                        }
-                        dropped = fminf(dropped, xd);
-                        thr = fminf(band_sq(cd[0], tau1, babs), init_thr);
-                    }
-                }
+                if (ncand < kCand) cand[ncand][threadIdx.x] = k;
+                ncand++;
             }
         }
         // pop the nearest pending subtree some lane still needs
         node = kNone;
         while (sp > 0) {
             --sp;
-            const StackEnt e = stk[sp];
-            const float dd = box_d2_fast(qx, qy, qz, e.mn, e.mx);
-            if (__ballot(dd <= thr) != 0) {
-                node = uniform_i(e.node);
+            const int e = (STACK == 64 || sp < 64) ? __builtin_amdgcn_readlane(stackv, sp & 63)
+                                                   : __builtin_amdgcn_readlane(stackv_hi, sp & 63);
+            const float *pb = (const float *)(nodes + (e >> 1)) + 6 * (e & 1);
+            const float mn[3] = {pb[0], pb[1], pb[2]}, mx[3] = {pb[3], pb[4], pb[5]};
+            const float dd = box_d2_fast(qx, qy, qz, mn, mx);
+            if (__ballot(FIX ? dd <= thr : dd < best) != 0) {
+                node = uniform_i(((const int *)(nodes + (e >> 1)))[12 + (e & 1)]);
                 break;
             }
         }
-        if (node == kNone) break;
+        if (node == kNone || overflow) break;
+        if (++iters > max_iters) break;
     }
+    if (lane == 0 && (overflow || iters > max_iters)) atomicOr(a.err_flag, overflow ? 1 : 2);
+    if (!FIX) bestd = best;
     if (!valid) return;
 
     // ---- resolve the closest triangle ----
-    int k_final = -1;      // leaf position
+    int k_final = bk;      // leaf position
     bool have64 = false;   // distance known in FP64
     double d2_64 = 0.0;
-    float d2_32 = SSDF_INF;
-    if (!FIX) {
-        k_final = bk;
-        d2_32 = bd;
-    } else {
-        const float bt = fminf(band_sq(cd[0], tau1, babs), init_thr);
-        int nb = 0;
-#pragma unroll
-        for (int i = 0; i < kCand; i++) nb += (ck[i] >= 0 && cd[i] <= bt) ? 1 : 0;
-        k_final = nb > 0 ? ck[0] : -1;
-        d2_32 = nb > 0 ? cd[0] : SSDF_INF;
-        if (dropped < SSDF_INF && dropped <= bt) {
-            // the exact kernel recomputes this query from the untouched output (its seed)
-            const unsigned pos = atomicAdd(a.fix_count, 1u);
-            if (pos < a.fix_cap) a.fix_list[pos] = (unsigned)qid;
-            return;
-        }
-        if (nb >= 2) {
+    if (FIX && bk >= 0) {
+        bool exact_needed = ncand > kCand;
+        if (!exact_needed && (ncand >= 2 || (tris[bk].f.flags & 1))) {
             const double px = qx, py = qy, pz = qz;
             double bestd = __builtin_inf();
             int bestt = INT_MAX, bestk = -1;
-#pragma unroll
-            for (int i = 0; i < kCand; i++) {
-                if (ck[i] >= 0 && cd[i] <= bt) {
-                    const TriF tr = tris[ck[i]].f;
-                    const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
-                                                       tr.b[2], tr.c[0], tr.c[1], tr.c[2]);
-                    if (d < bestd || (d == bestd && tr.orig < bestt)) bestd = d, bestt = tr.orig, bestk = ck[i];
-                }
+            for (int i = 0; i < ncand; i++) {
+                const int kc = cand[i][threadIdx.x];
+                const TriF tr = tris[kc].f;
+                const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2],
+                                                   tr.c[0], tr.c[1], tr.c[2]);
+                if (d < bestd || (d == bestd && tr.orig < bestt)) bestd = d, bestt = tr.orig, bestk = kc;
             }
             if (bestk >= 0) {
                 k_final = bestk;
                 have64 = true;
                 d2_64 = bestd;
+            } else {
+                exact_needed = true;  // FP64 rejects every candidate (degenerate triangles)
             }
+        }
+        if (exact_needed) {
+            // the exact kernel recomputes this query from the untouched output (its seed), searching
+            // only the FP32 band: the FP64 minimum cannot lie outside it
+            const unsigned pos = atomicAdd(a.fix_count, 1u);
+            if (pos < a.fix_cap) {
+                a.fix_list[2 * pos] = (unsigned)qid;
+                a.fix_list[2 * pos + 1] = __float_as_uint(band_sq(bestd, tau1, babs));
+            }
+            return;
         }
     }
 
-    const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(d2_32);
+    const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(bestd);
     if (OUT == OUT_EDF) {
         if (k_final >= 0) {
-            const float o = fminf(out_init, dist);
-            ((float *)a.out_d)[qid] = o;
+            ((float *)a.out_d)[qid] = fminf(out_init, dist);
             if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
         }
         return;
@@ -355,7 +325,7 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
     const float dot32 = sign_dot<float>(qx, qy, qz, tr.a, tr.b, tr.c, tr.orig, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
                                         a.vertex_n, nlen32);
     bool neg = dot32 < 0.0f;
-    if (dist < a.sign_eps || fabsf(dot32) <= (1e-3f * dist + 4.0f * babs) * nlen32) {
+    if ((a.flags & QF_SIGN_FP64) || dist < a.sign_eps || fabsf(dot32) <= (1e-3f * dist + 4.0f * babs) * nlen32) {
         const double A[3] = {tr.a[0], tr.a[1], tr.a[2]}, B[3] = {tr.b[0], tr.b[1], tr.b[2]},
                      C[3] = {tr.c[0], tr.c[1], tr.c[2]};
         double nl64;
@@ -399,8 +369,10 @@ __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
                                   : (QSRC == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz);
     for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
          item += (long long)gridDim.x * blockDim.x) {
+        double band2 = __builtin_inf();
         if (QUEUE) {
-            qid = (long long)a.fix_list[item];
+            qid = (long long)a.fix_list[2 * item];
+            band2 = (double)__uint_as_float(a.fix_list[2 * item + 1]);
         } else {
             qid = item;
         }
@@ -420,7 +392,7 @@ __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
         int bt = INT_MAX, bk = -1;
         const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
         {
-            const double b0 = sqrt(init2) + delta;
+            const double b0 = sqrt(fmin(init2, band2)) + delta;
             bound = b0 * b0;
         }
         int st_node[kExactStack];
@@ -590,6 +562,7 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
         return SSDF_ERR_UNSUPPORTED;
     }
     const bool fix = (flags & QF_FIXUP) != 0;
+    a.flags = flags;
     const int depth = m.hmeta.maxdepth;
     unsigned nwaves;
     if (qsrc == QSRC_POINTS) {

@@ -51,6 +51,7 @@ struct QueryArgs {
     int *err_flag;  // bit 1: a traversal stack would overflow; bit 2: iteration cap hit
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
     unsigned nblocks;  // launched blocks (XCD remap)
+    unsigned flags;    // QF_* of the call
 };
 
 int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,

@@ -298,3 +298,21 @@ def test_errors_are_reported(gpu_lib):
     o = S.DeviceArray(8, np.float32)
     with pytest.raises(S.SsdfError):
         rm.query_grid(S.MODE_SDF, (2, 2, 2), (-1, -1, -1), (1, 1, 1), o)  # signed needs normals
+
+
+def test_moving_mesh_update_vertices(oracle):
+    """A resident handle rebuilt in place after its vertices move answers for the moved mesh."""
+    m0 = mg.torus(0.6, 0.2, 80, 32, sigma=0.004, seed=3)
+    m1 = mg.torus(0.6, 0.2, 80, 32, sigma=0.004, seed=4)  # same topology, other vertices
+    rm = S.ResidentMesh.from_mesh(m0)
+    n, lo, hi = (24, 24, 12), (-1, -1, -0.4), (1, 1, 0.4)
+    x, y, z = mg.grid_points(*n, lo, hi)
+    for m in (m1, m0, m1):
+        rm.update_vertices(m.sx, m.sy, m.sz)
+        o = S.DeviceArray(x.size, np.float32)
+        t = S.DeviceArray(x.size, np.int32)
+        rm.query_grid(S.MODE_SDF, n, lo, hi, o, t, flags=S.FLAG_FIXUP)
+        s, tri, _, _, _ = oracle.sdf(x, y, z, m)
+        assert np.array_equal(t.download(), tri)
+        assert np.abs(o.download() - s).max() <= 1e-6 * 2
+    assert rm.info()["open_edges"] == 0
