@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mqueries/s @256^3×200k tris (FP32+fixup vs plain); served p50/p99 latency"
 FLOPS_BOX, FLOPS_TRI, FLOPS_SIGN = 8, 81, 8
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
 
 
 def log(*a):
@@ -154,15 +155,33 @@ def cpu_baseline(args, m, n, lo, hi):
                       f"edf_celllist<float> restated in C, OpenMP, {dt:.2f} s incl. its cell build"}
 
 
-def roofline(query_ms, n_queries):
+def measured_traffic(kernel):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it reads for a copy kernel of known size)."""
+    f = ROOT / "profiles" / "k_query_packet_torus200k_grid256_counters.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[2])
+    return None
+
+
+def roofline(query_ms, n_queries, nverts, ntris):
     wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["c2_torus_200k_grid256"]
     assert wc["queries"] == n_queries
     flops = FLOPS_BOX * wc["box_tests"] + FLOPS_TRI * wc["tri_tests"] + FLOPS_SIGN * n_queries
     achieved = flops / (query_ms * 1e-3) / 1e12
+    # grid SDF mode: 8 B per query (signed distance + triangle) plus one read of the mesh: vertices
+    # 12 B, index triples 12 B, FP32 BVH nodes 32 B x (2T - 1), FP32 triangle corners 36 B
+    hbm_bytes = 8 * n_queries + 12 * nverts + 12 * ntris + 32 * (2 * ntris - 1) + 36 * ntris
+    hbm = hbm_bytes / (query_ms * 1e-3) / 1e9
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_query_packet (FP32+fixup)",
-            "algorithmic_gflop_per_launch": round(flops / 1e9, 2),
-            "hbm_algorithmic_bytes_per_launch": 8 * n_queries}
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic("k_query_packet_fixup"),
+            "kernel": "k_query_packet (FP32+fixup)", "algorithmic_gflop_per_launch": round(flops / 1e9, 2),
+            "hbm": {"achieved": round(hbm, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hbm / PEAK_HBM_GBS, 5),
+                    "algorithmic_bytes_per_launch": hbm_bytes}}
 
 
 def main():
@@ -171,7 +190,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fixup", "plain"], default=None,
                     help="time one mode only and print nothing else (for profiling)")
@@ -212,7 +231,7 @@ def main():
         "served": None,
     }
     if args.grid == 256:
-        line["roofline"] = roofline(g["fixup"]["query_ms"], N)
+        line["roofline"] = roofline(g["fixup"]["query_ms"], N, m.nverts, m.ntris)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
     print(json.dumps(line), flush=True)

@@ -254,7 +254,7 @@ int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 

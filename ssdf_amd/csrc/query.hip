@@ -4,15 +4,15 @@
 // consecutive points) walks the tree as one: the node index is wave-uniform, so each 64-byte node
 // record arrives with one scalar load, every lane tests its own query against both child boxes,
 // and two ballots decide whether to descend left, right or both (nearest-first by majority vote,
-// the far child's box pushed on a wave-uniform LDS stack).  Popped entries are re-tested against
-// each lane's current bound before the node is fetched again.  Leaves hold one triangle; the whole
+// the far child pushed on a wave-uniform stack held one entry per lane in a VGPR).  Popped entries
+// are re-tested against each lane's current bound before the node is fetched again.  Leaves hold one triangle; the whole
 // wave evaluates it (point_triangle_dist_sq, serial/ssdf.hpp:50-121, in FP32).
 //
 // FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
 // FP32 error bound of a distance, and each lane lists the triangles inside that band.  After
 // traversal, a lane whose band holds one triangle is final; two or more are re-evaluated in FP64
 // (same formula, contraction off -> the oracle's bits) and the lowest index wins exact ties; a lane
-// whose list overflowed is queued for k_query_exact (full FP64 traversal).
+// whose list overflowed is queued for k_query_exact (FP64 traversal bounded by the lane's band).
 //
 // k_query_exact: one lane per query, FP64 box and triangle tests, private stack; exact argmin with
 // the canonical tie-break.  Used for FP64 meshes and for the fixup queue.
@@ -24,7 +24,7 @@
 namespace ssdf_amd {
 
 constexpr int kWaves = 4;
-constexpr int kCand = 16;
+constexpr int kCand = 24;  // 24 KB of LDS per block: six blocks per CU, as the FP32+fixup VGPR count allows
 constexpr int kNone = INT_MIN;
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -135,13 +135,16 @@ __device__ __forceinline__ void store_as(void *p, long long i, T v) {
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).
 //
-// Stack: wave-uniform ints in LDS, each naming where the pending subtree hangs (parent << 1 | side);
-// a pop re-reads that child's box from the parent record with scalar loads and re-tests it.
+// Stack: wave-uniform ints, entry i in lane i % 64 of a VGPR, each naming where the pending subtree
+// hangs (parent << 1 | side); a pop re-reads that child's box from the parent record with scalar
+// loads and re-tests it.
 // FIX candidates: per lane, an append-only list of leaf positions in LDS ([kCand][256], lane-major,
 // conflict-free).  A triangle enters when its FP32 d2 is inside band(best); a new best whose own
 // band lies below the old best empties the list first (every old entry is >= the old best, so
-// none of them can be in the new band).  The list therefore always holds every triangle inside the
-// current band, plus possibly a few outside it, which the FP64 evaluation rejects.
+// none of them can be in the new band), and a full list drops the entries outside the current
+// band.  The list therefore always holds every triangle inside the current band, plus possibly a
+// few outside it, which the FP64 evaluation rejects.  A list that stays full after the drop (more
+// than kCand triangles inside the band) sends the query to the exact FP64 traversal.
 // ------------------------------------------------------------------------------------------------
 template <int STACK, int QSRC, int OUT, bool FIX>
 __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
@@ -221,8 +224,20 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
                     bk = k;
                     thr = fminf(nb, init_thr);
                 }
-                if (ncand < kCand) cand[ncand][threadIdx.x] = k;
-                ncand++;
+                if (ncand < kCand) {
+                    cand[ncand][threadIdx.x] = k;
+                    ncand++;
+                } else if (ncand == kCand) {
+                    // full: drop the entries the band has shrunk past (re-evaluated by the same FP32
+                    // routine on the same inputs, so with the same values they entered with)
+                    int keep = 0;
+                    for (int i = 0; i < kCand; i++) {
+                        const int kc = cand[i][threadIdx.x];
+                        if (tri_d2_fast(qx, qy, qz, tris[kc]) <= thr) cand[keep++][threadIdx.x] = kc;
+                    }
+                    if (keep < kCand) cand[keep][threadIdx.x] = k;
+                    ncand = keep + 1;  // kCand + 1: more than kCand inside the band, exact path
+                }
             }
         }
         // pop the nearest pending subtree some lane still needs
