@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 23)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fixup", "plain"], default=None,
                     help="time one mode only and print nothing else (for profiling)")
