@@ -168,4 +168,61 @@ int sample_sdf_bvh(const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0, 
                                ymax, zmax, out);
 }
 
+namespace detail {
+inline int compute_aabb_impl(ptrdiff_t n, const float *x, const float *y, const float *z, float *b) {
+    return ssdf_compute_aabb_f(n, x, y, z, &b[0], &b[1], &b[2], &b[3], &b[4], &b[5]);
+}
+inline int compute_aabb_impl(ptrdiff_t n, const double *x, const double *y, const double *z, double *b) {
+    return ssdf_compute_aabb_d(n, x, y, z, &b[0], &b[1], &b[2], &b[3], &b[4], &b[5]);
+}
+inline int box_sdist_impl(ptrdiff_t n, const float *x, const float *y, const float *z, const float *b, float *out) {
+    return ssdf_all_points_aabb_signed_distance_f(n, x, y, z, b[0], b[1], b[2], b[3], b[4], b[5], out);
+}
+inline int box_sdist_impl(ptrdiff_t n, const double *x, const double *y, const double *z, const double *b,
+                          double *out) {
+    return ssdf_all_points_aabb_signed_distance_d(n, x, y, z, b[0], b[1], b[2], b[3], b[4], b[5], out);
+}
+// T(x[i]) for every point, as the reference converts before computing (serial/edf/edf.hpp:515-517)
+template <typename T, typename G>
+std::vector<T> widen(ptrdiff_t n, const G *v) {
+    return std::vector<T>(v, v + (n > 0 ? n : 0));
+}
+}  // namespace detail
+
+/// serial/edf/edf.hpp:500-539: bounds of npoints >= 1 points, computed in T on the GPU.
+template <typename G, typename T>
+int compute_aabb(const ptrdiff_t npoints, const G *const SSDF_RESTRICT x, const G *const SSDF_RESTRICT y,
+                 const G *const SSDF_RESTRICT z, T *const SSDF_RESTRICT xmin, T *const SSDF_RESTRICT xmax,
+                 T *const SSDF_RESTRICT ymin, T *const SSDF_RESTRICT ymax, T *const SSDF_RESTRICT zmin,
+                 T *const SSDF_RESTRICT zmax) {
+    T b[6];
+    int rc;
+    if constexpr (std::is_same<G, T>::value) {
+        rc = detail::compute_aabb_impl(npoints, x, y, z, b);
+    } else {
+        const auto wx = detail::widen<T>(npoints, x), wy = detail::widen<T>(npoints, y),
+                   wz = detail::widen<T>(npoints, z);
+        rc = detail::compute_aabb_impl(npoints, wx.data(), wy.data(), wz.data(), b);
+    }
+    if (rc) return rc;
+    *xmin = b[0], *xmax = b[1], *ymin = b[2], *ymax = b[3], *zmin = b[4], *zmax = b[5];
+    return 0;
+}
+
+/// serial/edf/edf.hpp:541-557: signed distance of every point to the box (negative inside), in T.
+template <typename G, typename T>
+int all_points_aabb_signed_distance(const ptrdiff_t npoints, const G *const SSDF_RESTRICT x,
+                                    const G *const SSDF_RESTRICT y, const G *const SSDF_RESTRICT z, const T minx,
+                                    const T maxx, const T miny, const T maxy, const T minz, const T maxz,
+                                    T *const SSDF_RESTRICT out) {
+    const T b[6] = {minx, maxx, miny, maxy, minz, maxz};
+    if constexpr (std::is_same<G, T>::value) {
+        return detail::box_sdist_impl(npoints, x, y, z, b, out);
+    } else {
+        const auto wx = detail::widen<T>(npoints, x), wy = detail::widen<T>(npoints, y),
+                   wz = detail::widen<T>(npoints, z);
+        return detail::box_sdist_impl(npoints, wx.data(), wy.data(), wz.data(), b, out);
+    }
+}
+
 }  // namespace ssdf

@@ -89,6 +89,33 @@ int ssdf_sdf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z
                const float *sz, float *out_signed, int *out_tri, unsigned flags);
 
 /* ---------------------------------------------------------------------------------------------
+ * Seeding helpers of the distance driver (host pointers, synchronous).
+ * --------------------------------------------------------------------------------------------- */
+
+/* compute_aabb (serial/edf/edf.hpp:500-539): bounds of npoints >= 1 points, seeded with point 0 and
+ * reduced with the reference's min/max.  SSDF_ERR_INVALID for npoints <= 0 (the reference reads
+ * x[0]). */
+int ssdf_compute_aabb_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, float *xmin, float *xmax,
+                        float *ymin, float *ymax, float *zmin, float *zmax);
+int ssdf_compute_aabb_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, double *xmin,
+                        double *xmax, double *ymin, double *ymax, double *zmin, double *zmax);
+
+/* all_points_aabb_signed_distance (serial/edf/edf.hpp:541-557 over point_aabb_signed_distance,
+ * 193-221): signed distance of every point to the box, negative inside; overwrites out. */
+int ssdf_all_points_aabb_signed_distance_f(ptrdiff_t npoints, const float *x, const float *y, const float *z,
+                                           float minx, float maxx, float miny, float maxy, float minz, float maxz,
+                                           float *out);
+int ssdf_all_points_aabb_signed_distance_d(ptrdiff_t npoints, const double *x, const double *y, const double *z,
+                                           double minx, double maxx, double miny, double maxy, double minz,
+                                           double maxz, double *out);
+
+/* The SSDF_INIT_WITH_VOLUME_BOX seed of drivers/sdf.exe.cpp:317-327 on DEVICE arrays, asynchronous
+ * on `stream`: out[i] = |signed distance of point i to the bounds of all points|, the initial value
+ * of `out` for the edf entry points.  dtype 0 float, 1 double; points and out in that precision. */
+int ssdf_init_volume_box(int dtype, ptrdiff_t npoints, const void *x, const void *y, const void *z, void *out,
+                         void *stream);
+
+/* ---------------------------------------------------------------------------------------------
  * Resident meshes (no reference counterpart: the reference rebuilds its BVH on every call,
  * serial/bvh/bvh.cu:80-158).  A mesh handle keeps its LBVH (and pseudonormals) in HBM; queries take
  * DEVICE pointers and run asynchronously on `stream` (a hipStream_t, NULL = default stream).

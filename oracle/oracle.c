@@ -277,7 +277,7 @@ void orc_bvh_argmin(ptrdiff_t npoints, const double *x, const double *y, const d
         L = fmax(L, fabs(sy[v]));
         L = fmax(L, fabs(sz[v]));
     }
-    float *fmin_ = (float *)malloc(sizeof(float) * 3 * nsel), *fmax_ = (float *)malloc(sizeof(float) * 3 * nsel);
+    float *fmin_ = (float *)calloc(3 * nsel + 1, sizeof(float)), *fmax_ = (float *)calloc(3 * nsel + 1, sizeof(float));
     double *lbox = (double *)malloc(sizeof(double) * 6 * nsel);
     for (ptrdiff_t t = 0; t < nsel; t++) {
         const int iv[3] = {s0[t], s1[t], s2[t]};
@@ -375,7 +375,7 @@ void orc_bvh_argmin(ptrdiff_t npoints, const double *x, const double *y, const d
 void orc_lbvh_counts(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nsel,
                      const int *s0, const int *s1, const int *s2, const float *sx, const float *sy, const float *sz,
                      int64_t *out_nbox, int64_t *out_ntri, int *out_maxdepth, float *out_d2) {
-    float *fmin_ = (float *)malloc(sizeof(float) * 3 * nsel), *fmax_ = (float *)malloc(sizeof(float) * 3 * nsel);
+    float *fmin_ = (float *)calloc(3 * nsel + 1, sizeof(float)), *fmax_ = (float *)calloc(3 * nsel + 1, sizeof(float));
     for (ptrdiff_t t = 0; t < nsel; t++) {
         const int iv[3] = {s0[t], s1[t], s2[t]};
         const float *co[3] = {sx, sy, sz};

@@ -32,6 +32,14 @@ float orc_pt_box_sdist_f(float px, float py, float pz, float minx, float maxx, f
                          float maxz);
 double orc_pt_box_sdist_d(double px, double py, double pz, double minx, double maxx, double miny, double maxy,
                           double minz, double maxz);
+/* compute_aabb / all_points_aabb_signed_distance (serial/edf/edf.hpp:500-557); box6 = xmin, xmax,
+ * ymin, ymax, zmin, zmax */
+void orc_compute_aabb_f(ptrdiff_t n, const float *x, const float *y, const float *z, float *box6);
+void orc_compute_aabb_d(ptrdiff_t n, const double *x, const double *y, const double *z, double *box6);
+void orc_all_points_aabb_sdist_f(ptrdiff_t n, const float *x, const float *y, const float *z, const float *box6,
+                                 float *out);
+void orc_all_points_aabb_sdist_d(ptrdiff_t n, const double *x, const double *y, const double *z, const double *box6,
+                                 double *out);
 
 /* ---- the reference's two CPU engines, restated (orc_real.h) ---- */
 int orc_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nsel, const int *s0,

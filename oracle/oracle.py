@@ -32,6 +32,9 @@ def _load():
             "orc_pt_box_d2_f": ([C.c_float] * 9, C.c_float), "orc_pt_box_d2_d": ([C.c_double] * 9, C.c_double),
             "orc_pt_box_sdist_f": ([C.c_float] * 9, C.c_float),
             "orc_pt_box_sdist_d": ([C.c_double] * 9, C.c_double),
+            "orc_compute_aabb_f": ([_PD, _P, _P, _P, _P], None), "orc_compute_aabb_d": ([_PD, _P, _P, _P, _P], None),
+            "orc_all_points_aabb_sdist_f": ([_PD, _P, _P, _P, _P, _P], None),
+            "orc_all_points_aabb_sdist_d": ([_PD, _P, _P, _P, _P, _P], None),
             "orc_edf_f": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P], C.c_int),
             "orc_edf_d": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P], C.c_int),
             "orc_celllist_f": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P], C.c_int),
@@ -118,6 +121,29 @@ def pt_box_sdist(p, lo, hi, dtype=np.float64):
     L = _load()
     f = L.orc_pt_box_sdist_d if dtype == np.float64 else L.orc_pt_box_sdist_f
     return f(p[0], p[1], p[2], lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+
+
+def compute_aabb(x, y, z):
+    """(xmin, xmax, ymin, ymax, zmin, zmax) in the points' precision (serial/edf/edf.hpp:500-539)."""
+    L = _load()
+    dt = np.asarray(x).dtype
+    x, y, z = (np.ascontiguousarray(v, dtype=dt) for v in (x, y, z))
+    box = np.empty(6, dt)
+    f = L.orc_compute_aabb_d if dt == np.float64 else L.orc_compute_aabb_f
+    f(x.size, _p(x), _p(y), _p(z), _p(box))
+    return box
+
+
+def all_points_aabb_sdist(x, y, z, box6):
+    """Signed distance of each point to box6 (serial/edf/edf.hpp:541-557), in the points' precision."""
+    L = _load()
+    dt = np.asarray(x).dtype
+    x, y, z = (np.ascontiguousarray(v, dtype=dt) for v in (x, y, z))
+    box = np.ascontiguousarray(box6, dtype=dt)
+    out = np.empty(x.size, dt)
+    f = L.orc_all_points_aabb_sdist_d if dt == np.float64 else L.orc_all_points_aabb_sdist_f
+    f(x.size, _p(x), _p(y), _p(z), _p(box), _p(out))
+    return out
 
 
 def _engine(name, x, y, z, mesh, out):

@@ -159,6 +159,26 @@ REAL FN(orc_pt_box_sdist)(const REAL px, const REAL py, const REAL pz, const REA
     return outside + inside;
 }
 
+/* Bounds of a point cloud (npoints >= 1), seeded with point 0 and reduced by
+ * min(a, b) = b < a ? b : a, max(a, b) = a < b ? b : a; follows serial/edf/edf.hpp:500-539.
+ * box6 = xmin, xmax, ymin, ymax, zmin, zmax. */
+void FN(orc_compute_aabb)(const ptrdiff_t n, const REAL *x, const REAL *y, const REAL *z, REAL *box6) {
+    REAL b[6] = {x[0], x[0], y[0], y[0], z[0], z[0]};
+    for (ptrdiff_t i = 0; i < n; i++) {
+        b[0] = ORC_MIN(b[0], x[i]), b[1] = ORC_MAX(b[1], x[i]);
+        b[2] = ORC_MIN(b[2], y[i]), b[3] = ORC_MAX(b[3], y[i]);
+        b[4] = ORC_MIN(b[4], z[i]), b[5] = ORC_MAX(b[5], z[i]);
+    }
+    for (int k = 0; k < 6; k++) box6[k] = b[k];
+}
+
+/* Signed distance of every point to a box; follows serial/edf/edf.hpp:541-557. */
+void FN(orc_all_points_aabb_sdist)(const ptrdiff_t n, const REAL *x, const REAL *y, const REAL *z, const REAL *box6,
+                                   REAL *out) {
+    for (ptrdiff_t i = 0; i < n; i++)
+        out[i] = FN(orc_pt_box_sdist)(x[i], y[i], z[i], box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]);
+}
+
 REAL FN(orc_pt_box_d2)(const REAL px, const REAL py, const REAL pz, const REAL minx, const REAL maxx,
                        const REAL miny, const REAL maxy, const REAL minz, const REAL maxz) {
     return FN(pt_box_d2)(px, py, pz, minx, maxx, miny, maxy, minz, maxz);

@@ -57,6 +57,11 @@ _SIGS = {
     "ssdf_sample_sdf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double,
                           C.c_double, C.c_double, C.c_double, _P, _P],
     "ssdf_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, C.c_uint],
+    "ssdf_compute_aabb_f": [_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "ssdf_compute_aabb_d": [_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "ssdf_all_points_aabb_signed_distance_f": [_PD, _P, _P, _P] + [C.c_float] * 6 + [_P],
+    "ssdf_all_points_aabb_signed_distance_d": [_PD, _P, _P, _P] + [C.c_double] * 6 + [_P],
+    "ssdf_init_volume_box": [C.c_int, _PD, _P, _P, _P, _P, _P],
     "ssdf_mesh_create": [C.c_int, _PD, _P, _P, _P, _PD, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
     "ssdf_mesh_destroy": [_P],
     "ssdf_mesh_rebuild": [_P, C.c_int, _P],
@@ -139,6 +144,10 @@ class DeviceArray:
         p = C.c_void_p()
         _check(lib().ssdf_device_malloc(C.byref(p), max(1, self.n * self.dtype.itemsize)), "ssdf_device_malloc")
         self.ptr = p.value
+
+    @property
+    def size(self) -> int:
+        return self.n
 
     @classmethod
     def from_host(cls, a: np.ndarray) -> "DeviceArray":
@@ -288,6 +297,42 @@ def sdf(x, y, z, s0, s1, s2, sx, sy, sz, out_signed, out_tri=None, flags: int = 
                           _ptr(sx), _ptr(sy), _ptr(sz), _ptr(out_signed), _ptr(out_tri), flags)
     _check(rc, "ssdf_sdf_f")
     return 0
+
+
+def compute_aabb(x, y, z):
+    """ssdf::compute_aabb (serial/edf/edf.hpp:500-539): (xmin, xmax, ymin, ymax, zmin, zmax) of the
+    points, in their precision."""
+    p = _prec(x, y, z)
+    T = C.c_float if p == "f" else C.c_double
+    b = [T() for _ in range(6)]
+    fn = lib().ssdf_compute_aabb_f if p == "f" else lib().ssdf_compute_aabb_d
+    rc = fn(x.size, _ptr(x), _ptr(y), _ptr(z), *[C.byref(v) for v in b])
+    _check(rc, f"ssdf_compute_aabb_{p}")
+    return tuple(v.value for v in b)
+
+
+def all_points_aabb_signed_distance(x, y, z, minx, maxx, miny, maxy, minz, maxz, out) -> int:
+    """ssdf::all_points_aabb_signed_distance (serial/edf/edf.hpp:541-557): signed distance of every
+    point to the box (negative inside), overwriting `out`."""
+    p = _prec(x, y, z, out)
+    fn = lib().ssdf_all_points_aabb_signed_distance_f if p == "f" else lib().ssdf_all_points_aabb_signed_distance_d
+    rc = fn(x.size, _ptr(x), _ptr(y), _ptr(z), minx, maxx, miny, maxy, minz, maxz, _ptr(out))
+    _check(rc, f"ssdf_all_points_aabb_signed_distance_{p}")
+    return 0
+
+
+def init_volume_box(x: "DeviceArray", y: "DeviceArray", z: "DeviceArray", out: "DeviceArray",
+                    stream: "Stream | None" = None) -> None:
+    """The SSDF_INIT_WITH_VOLUME_BOX seed (drivers/sdf.exe.cpp:317-327) on device arrays:
+    out = |signed distance to the bounds of all points|, asynchronous on `stream`."""
+    dt = {np.dtype(a.dtype) for a in (x, y, z, out)}
+    if len(dt) != 1 or dt.pop() not in (np.dtype(np.float32), np.dtype(np.float64)):
+        raise TypeError("points and out must share one float dtype")
+    if not (x.size == y.size == z.size == out.size):
+        raise ValueError("x, y, z and out must have the same length")
+    rc = lib().ssdf_init_volume_box(0 if x.dtype == np.float32 else 1, x.size, _ptr(x), _ptr(y), _ptr(z), _ptr(out),
+                                    C.c_void_p(stream.ptr) if stream is not None else None)
+    _check(rc, "ssdf_init_volume_box")
 
 
 # ------------------------------------------------------------------------------------------------

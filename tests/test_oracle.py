@@ -406,3 +406,40 @@ def test_lbvh_replay_distances(oracle):
     assert np.abs(np.sqrt(d2f.astype(np.float64)) - np.sqrt(d2)).max() < 1e-6
     assert 0 < md < 64
     assert nb.min() >= 2 and nt.min() >= 1
+
+
+# ----------------------------------------------------------------------------------------------
+# the driver's seeding helpers (serial/edf/edf.hpp:500-557)
+# ----------------------------------------------------------------------------------------------
+def test_compute_aabb_and_box_sdist_known_answers(oracle):
+    # the 8 corners of [-1, 2] x [0, 1] x [3, 5] plus interior points: bounds are the corners
+    c = np.array([[x, y, z] for x in (-1, 2) for y in (0, 1) for z in (3, 5)], float)
+    pts = np.vstack([c, [[0.5, 0.5, 4.0], [0, 0.25, 3.5]]])
+    for dt in (np.float64, np.float32):
+        x, y, z = (np.ascontiguousarray(pts[:, d], dtype=dt) for d in range(3))
+        box = oracle.compute_aabb(x, y, z)
+        assert list(box) == [-1, 2, 0, 1, 3, 5]
+        s = oracle.all_points_aabb_sdist(x, y, z, box)
+        assert np.all(s[:8] == 0)  # corners lie on the box
+        assert s[8] == -0.5  # centre: half the smallest extent inside
+        assert s[9] == -0.25  # 0.25 from the y = 0 face, 0.5 from z = 3, 1 from x = -1
+    # outside: distance to the nearest corner
+    x, y, z = np.array([3.0]), np.array([2.0]), np.array([6.0])
+    assert oracle.all_points_aabb_sdist(x, y, z, np.array([-1, 2, 0, 1, 3, 5.0]))[0] == np.sqrt(3.0)
+
+
+def test_compute_aabb_matches_reference_header(oracle):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference header shim not built (no /root/reference)")
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 17, 1000):
+        x, y, z = rng.normal(size=(3, n)) * rng.uniform(0.1, 10, (3, 1))
+        x, y, z = (np.ascontiguousarray(v) for v in (x, y, z))
+        box = np.empty(6)
+        out = np.empty(n)
+        p = lambda a: C.c_void_p(a.ctypes.data)
+        ref.ref_all_points_aabb_signed_distance_d(n, p(x), p(y), p(z), p(box), p(out))
+        assert np.array_equal(oracle.compute_aabb(x, y, z), box)
+        assert np.array_equal(oracle.all_points_aabb_sdist(x, y, z, box), out)
