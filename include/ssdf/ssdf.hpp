@@ -169,6 +169,54 @@ int sample_sdf_bvh(const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0, 
 }
 
 namespace detail {
+inline int inside_points_impl(ptrdiff_t n, const float *x, const float *y, const float *z, ptrdiff_t ns,
+                              const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const float *sx,
+                              const float *sy, const float *sz, uint8_t *out) {
+    return ssdf_points_inside_f(n, x, y, z, ns, s0, s1, s2, nv, sx, sy, sz, out);
+}
+inline int inside_points_impl(ptrdiff_t n, const double *x, const double *y, const double *z, ptrdiff_t ns,
+                              const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const double *sx,
+                              const double *sy, const double *sz, uint8_t *out) {
+    return ssdf_points_inside_d(n, x, y, z, ns, s0, s1, s2, nv, sx, sy, sz, out);
+}
+inline int inside_grid_impl(ptrdiff_t ns, const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const float *sx,
+                            const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, float x0,
+                            float y0, float z0, float x1, float y1, float z1, uint8_t *out) {
+    return ssdf_sample_inside_f(ns, s0, s1, s2, nv, sx, sy, sz, nx, ny, nz, x0, y0, z0, x1, y1, z1, out);
+}
+inline int inside_grid_impl(ptrdiff_t ns, const int *s0, const int *s1, const int *s2, ptrdiff_t nv,
+                            const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                            ptrdiff_t nz, double x0, double y0, double z0, double x1, double y1, double z1,
+                            uint8_t *out) {
+    return ssdf_sample_inside_d(ns, s0, s1, s2, nv, sx, sy, sz, nx, ny, nz, x0, y0, z0, x1, y1, z1, out);
+}
+}  // namespace detail
+
+/// serial/bvh/bvh.hpp:53-66: out[i] = 1 inside the surface, else 0 (generalized winding number >= 1/2,
+/// the ray-parity decision of the reference on closed surfaces).
+template <typename G, typename I>
+int points_inside_bvh(const ptrdiff_t npoints, const G *const SSDF_RESTRICT x, const G *const SSDF_RESTRICT y,
+                      const G *const SSDF_RESTRICT z, const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0,
+                      const I *const SSDF_RESTRICT s1, const I *const SSDF_RESTRICT s2, const ptrdiff_t nspoints,
+                      const G *const SSDF_RESTRICT sx, const G *const SSDF_RESTRICT sy,
+                      const G *const SSDF_RESTRICT sz, uint8_t *const SSDF_RESTRICT out) {
+    (void)sizeof(detail::check_types<G, G, I>);
+    return detail::inside_points_impl(npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz, out);
+}
+
+/// serial/bvh/bvh.hpp:229-247: inside flags on the nodes of a grid, x fastest.
+template <typename G, typename I>
+int sample_inside_bvh(const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0, const I *const SSDF_RESTRICT s1,
+                      const I *const SSDF_RESTRICT s2, const ptrdiff_t nspoints, const G *const SSDF_RESTRICT sx,
+                      const G *const SSDF_RESTRICT sy, const G *const SSDF_RESTRICT sz, const ptrdiff_t nx,
+                      const ptrdiff_t ny, const ptrdiff_t nz, const G xmin, const G ymin, const G zmin, const G xmax,
+                      const G ymax, const G zmax, uint8_t *const SSDF_RESTRICT out) {
+    (void)sizeof(detail::check_types<G, G, I>);
+    return detail::inside_grid_impl(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                    ymax, zmax, out);
+}
+
+namespace detail {
 inline int compute_aabb_impl(ptrdiff_t n, const float *x, const float *y, const float *z, float *b) {
     return ssdf_compute_aabb_f(n, x, y, z, &b[0], &b[1], &b[2], &b[3], &b[4], &b[5]);
 }

@@ -32,6 +32,10 @@ enum {
 /* query flags */
 #define SSDF_FLAG_FIXUP 1u     /* FP32 traversal + FP64 re-evaluation of near-tied candidates */
 #define SSDF_FLAG_SIGN_FP64 2u /* evaluate every sign in FP64 */
+/* signs from the generalized winding number (negative where w >= 1/2) instead of pseudonormals:
+ * always (GWN), or only for meshes with open or non-manifold edges (AUTO) */
+#define SSDF_FLAG_SIGN_GWN 8u
+#define SSDF_FLAG_SIGN_AUTO 16u
 
 /* query modes of the resident-mesh API */
 #define SSDF_MODE_EDF 0     /* unsigned distance, min-accumulated into out_d */
@@ -87,6 +91,27 @@ int ssdf_sample_sdf_d(ptrdiff_t nselements, const int *s0, const int *s1, const 
 int ssdf_sdf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
                const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
                const float *sz, float *out_signed, int *out_tri, unsigned flags);
+
+/* points_inside_bvh (serial/bvh/bvh.hpp:53-66, bvh.cpp:318-365): out[i] = 1 if point i is inside
+ * the surface, else 0.  The reference decides by cuBQL's ray crossing parity; this engine by the
+ * generalized winding number w >= 1/2, the same decision on closed surfaces and a stable one on open
+ * ones.  An empty surface gives 0 everywhere. */
+int ssdf_points_inside_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx,
+                         const float *sy, const float *sz, uint8_t *out);
+int ssdf_points_inside_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const double *sx,
+                         const double *sy, const double *sz, uint8_t *out);
+
+/* sample_inside_bvh (serial/bvh/bvh.hpp:229-247, bvh.cpp:1800-1866): the same on the nodes of an
+ * nx x ny x nz grid over [min, max], x fastest. */
+int ssdf_sample_inside_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                         float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, uint8_t *out);
+int ssdf_sample_inside_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                         ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                         uint8_t *out);
 
 /* ---------------------------------------------------------------------------------------------
  * Seeding helpers of the distance driver (host pointers, synchronous).
@@ -158,6 +183,17 @@ int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const 
 int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,
                          double ymin, double zmin, double xmax, double ymax, double zmax, void *out_d, int *out_tri,
                          unsigned flags, void *stream);
+
+/* Generalized winding number w of each point (device arrays, mesh precision in, float out): 1 inside
+ * a closed outward-oriented surface, 0 outside, fractional near holes.  Exact solid angles near the
+ * point, order-1 far-field terms for subtrees beyond 3 radii. */
+int ssdf_mesh_winding_number(ssdf_mesh_t mesh, ptrdiff_t npoints, const void *x, const void *y, const void *z,
+                             float *out_w, void *stream);
+/* Inside flags (w >= 1/2) at points or on grid nodes (device arrays). */
+int ssdf_mesh_inside_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const void *x, const void *y, const void *z,
+                            uint8_t *out_inside, void *stream);
+int ssdf_mesh_inside_grid(ssdf_mesh_t mesh, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin, double ymin,
+                          double zmin, double xmax, double ymax, double zmax, uint8_t *out_inside, void *stream);
 
 /* Statistics of the last query on this handle (after its stream has completed): number of
  * queries re-traversed in FP64 because their candidate list overflowed, and the kernel error flag

@@ -21,11 +21,15 @@ import numpy as np
 __all__ = [
     "SsdfError", "lib", "available", "edf", "edf_celllist", "edf_select", "edf_bvh", "closest_point_bvh",
     "sample_sdf_bvh", "sdf", "ResidentMesh", "DeviceArray", "Stream", "Event", "FLAG_FIXUP", "FLAG_SIGN_FP64",
+    "FLAG_SIGN_GWN", "FLAG_SIGN_AUTO", "points_inside_bvh", "sample_inside_bvh", "compute_aabb",
+    "all_points_aabb_signed_distance", "init_volume_box",
     "MODE_EDF", "MODE_SDF", "MODE_CLOSEST",
 ]
 
 FLAG_FIXUP = 1
 FLAG_SIGN_FP64 = 2
+FLAG_SIGN_GWN = 8
+FLAG_SIGN_AUTO = 16
 MODE_EDF, MODE_SDF, MODE_CLOSEST = 0, 1, 2
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libssdf_amd.so"
@@ -57,6 +61,13 @@ _SIGS = {
     "ssdf_sample_sdf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double,
                           C.c_double, C.c_double, C.c_double, _P, _P],
     "ssdf_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, C.c_uint],
+    "ssdf_points_inside_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
+    "ssdf_points_inside_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
+    "ssdf_sample_inside_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD] + [C.c_float] * 6 + [_P],
+    "ssdf_sample_inside_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD] + [C.c_double] * 6 + [_P],
+    "ssdf_mesh_winding_number": [_P, _PD, _P, _P, _P, _P, _P],
+    "ssdf_mesh_inside_points": [_P, _PD, _P, _P, _P, _P, _P],
+    "ssdf_mesh_inside_grid": [_P, _PD, _PD, _PD] + [C.c_double] * 6 + [_P, _P],
     "ssdf_compute_aabb_f": [_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "ssdf_compute_aabb_d": [_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "ssdf_all_points_aabb_signed_distance_f": [_PD, _P, _P, _P] + [C.c_float] * 6 + [_P],
@@ -299,6 +310,33 @@ def sdf(x, y, z, s0, s1, s2, sx, sy, sz, out_signed, out_tri=None, flags: int = 
     return 0
 
 
+def points_inside_bvh(x, y, z, s0, s1, s2, sx, sy, sz, out) -> int:
+    """ssdf::points_inside_bvh (serial/bvh/bvh.hpp:53-66): out[i] = 1 inside the surface, else 0
+    (uint8), decided by the generalized winding number."""
+    p = _prec(x, y, z, sx, sy, sz)
+    _idx(s0, s1, s2)
+    if out.dtype != np.uint8 or out.size != x.size:
+        raise TypeError("out must be a uint8 array with one entry per point")
+    fn = lib().ssdf_points_inside_f if p == "f" else lib().ssdf_points_inside_d
+    rc = fn(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy),
+            _ptr(sz), _ptr(out))
+    _check(rc, f"ssdf_points_inside_{p}")
+    return 0
+
+
+def sample_inside_bvh(s0, s1, s2, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, out) -> int:
+    """ssdf::sample_inside_bvh (serial/bvh/bvh.hpp:229-247): inside flags on grid nodes, x fastest."""
+    p = _prec(sx, sy, sz)
+    _idx(s0, s1, s2)
+    if out.dtype != np.uint8 or out.size != nx * ny * nz:
+        raise TypeError("out must be a uint8 array of nx * ny * nz entries")
+    fn = lib().ssdf_sample_inside_f if p == "f" else lib().ssdf_sample_inside_d
+    rc = fn(s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy), _ptr(sz), nx, ny, nz, xmin, ymin,
+            zmin, xmax, ymax, zmax, _ptr(out))
+    _check(rc, f"ssdf_sample_inside_{p}")
+    return 0
+
+
 def compute_aabb(x, y, z):
     """ssdf::compute_aabb (serial/edf/edf.hpp:500-539): (xmin, xmax, ymin, ymax, zmin, zmax) of the
     points, in their precision."""
@@ -379,6 +417,23 @@ class ResidentMesh:
                                         float(lo[2]), float(hi[0]), float(hi[1]), float(hi[2]), _ptr(out_d),
                                         _ptr(out_tri), flags, C.c_void_p(stream.ptr) if stream else None)
         _check(rc, "ssdf_mesh_query_grid")
+
+    def winding_number(self, x, y, z, out_w, stream: Stream | None = None):
+        rc = lib().ssdf_mesh_winding_number(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_w),
+                                            C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_winding_number")
+
+    def inside_points(self, x, y, z, out_inside, stream: Stream | None = None):
+        rc = lib().ssdf_mesh_inside_points(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_inside),
+                                           C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_inside_points")
+
+    def inside_grid(self, n, lo, hi, out_inside, stream: Stream | None = None):
+        nx, ny, nz = n
+        rc = lib().ssdf_mesh_inside_grid(C.c_void_p(self.h), nx, ny, nz, float(lo[0]), float(lo[1]), float(lo[2]),
+                                         float(hi[0]), float(hi[1]), float(hi[2]), _ptr(out_inside),
+                                         C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_inside_grid")
 
     def brute_points(self, x, y, z, out_d2, out_tri, stream: Stream | None = None):
         rc = lib().ssdf_mesh_brute_points(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_d2),

@@ -203,6 +203,47 @@ void destroy_mesh(ssdf_mesh *h) {
     delete h;
 }
 
+// Grid nodes x_i = min + i h with h = (max - min) / (n - 1) in the mesh precision
+// (serial/bvh/bvh.cpp:1665-1667, 1744).
+void setup_grid(const DevMesh &m, QueryArgs &a, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, const double *lo,
+                const double *hi) {
+    a.nx = (int)nx, a.ny = (int)ny, a.nz = (int)nz;
+    const ptrdiff_t ns[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; d++) {
+        if (m.dtype == 0) {
+            const float l = (float)lo[d], u = (float)hi[d];
+            a.gmin[d] = l;
+            a.gh[d] = ns[d] > 1 ? (double)((u - l) / (float)(ns[d] - 1)) : 0.0;
+        } else {
+            a.gmin[d] = lo[d];
+            a.gh[d] = ns[d] > 1 ? (hi[d] - lo[d]) / (double)(ns[d] - 1) : 0.0;
+        }
+    }
+}
+
+// Signs from winding numbers: always with SSDF_FLAG_SIGN_GWN; with SSDF_FLAG_SIGN_AUTO when the
+// edge pass found open or non-manifold (same-direction) edges, where pseudonormals lose meaning.
+bool gwn_sign_wanted(const DevMesh &m, unsigned flags) {
+    if (flags & SSDF_FLAG_SIGN_GWN) return true;
+    return (flags & SSDF_FLAG_SIGN_AUTO) && (m.hmeta.open_edges > 0 || m.hmeta.flipped_edges > 0);
+}
+
+__global__ void k_fill_empty(size_t n, float *w, unsigned char *in) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (w) w[i] = 0.0f;
+        if (in) in[i] = 0;
+    }
+}
+
+// An empty surface encloses nothing: w = 0, outside everywhere.
+int fill_empty(int mode, size_t n, float *out_w, unsigned char *out_in, hipStream_t stream) {
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_empty, dim3(nb), dim3(256), 0, stream, n, mode == GW_WINDING ? out_w : nullptr,
+                       mode == GW_INSIDE ? out_in : nullptr);
+    SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
+}
+
 int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y, const void *z, void *out_d,
                  int *out_tri, void *cx, void *cy, void *cz, unsigned flags, hipStream_t stream) {
     if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
@@ -218,7 +259,10 @@ int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void 
     a.out_tri = out_tri;
     a.out_cx = cx, a.out_cy = cy, a.out_cz = cz;
     const int dbl = h->m.dtype;
-    return launch_query(h->m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream);
+    if (int rc = launch_query(h->m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream)) return rc;
+    if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags))
+        return launch_gwn(h->m, a, QSRC_POINTS, GW_SIGN, dbl, stream);
+    return SSDF_OK;
 }
 
 int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, const double *lo, const double *hi,
@@ -231,23 +275,49 @@ int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
     QueryArgs a{};
     const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
     if (int rc = prepare_args(h, a, nq, flags, stream)) return rc;
-    a.nx = (int)nx, a.ny = (int)ny, a.nz = (int)nz;
-    const ptrdiff_t ns[3] = {nx, ny, nz};
-    for (int d = 0; d < 3; d++) {
-        // spacing in the grid's precision (serial/bvh/bvh.cpp:1665-1667)
-        if (h->m.dtype == 0) {
-            const float l = (float)lo[d], u = (float)hi[d];
-            a.gmin[d] = l;
-            a.gh[d] = ns[d] > 1 ? (double)((u - l) / (float)(ns[d] - 1)) : 0.0;
-        } else {
-            a.gmin[d] = lo[d];
-            a.gh[d] = ns[d] > 1 ? (hi[d] - lo[d]) / (double)(ns[d] - 1) : 0.0;
-        }
-    }
+    setup_grid(h->m, a, nx, ny, nz, lo, hi);
     a.out_d = out_d;
     a.out_tri = out_tri;
     const int dbl = h->m.dtype;
-    return launch_query(h->m, a, QSRC_GRID, mode, flags, dbl, dbl, stream);
+    if (int rc = launch_query(h->m, a, QSRC_GRID, mode, flags, dbl, dbl, stream)) return rc;
+    if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags))
+        return launch_gwn(h->m, a, QSRC_GRID, GW_SIGN, dbl, stream);
+    return SSDF_OK;
+}
+
+// Winding numbers (GW_WINDING into out_w) or inside flags (GW_INSIDE into out_in) at points.
+int gwn_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y, const void *z, float *out_w,
+               unsigned char *out_in, hipStream_t stream) {
+    if (n <= 0) return SSDF_OK;
+    if (!x || !y || !z || (mode == GW_WINDING ? !out_w : !out_in)) {
+        set_error("winding-number query: null argument");
+        return SSDF_ERR_INVALID;
+    }
+    if (h->m.nsel <= 0) return fill_empty(mode, (size_t)n, out_w, out_in, stream);
+    QueryArgs a{};
+    if (int rc = prepare_args(h, a, (size_t)n, 0, stream)) return rc;
+    a.qx = x, a.qy = y, a.qz = z;
+    a.npoints = n;
+    a.out_w = out_w;
+    a.out_in = out_in;
+    return launch_gwn(h->m, a, QSRC_POINTS, mode, h->m.dtype, stream);
+}
+
+int gwn_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, const double *lo, const double *hi,
+             float *out_w, unsigned char *out_in, hipStream_t stream) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return SSDF_OK;
+    if (nx > 0x7fffffff || ny > 0x7fffffff || nz > 0x7fffffff || (mode == GW_WINDING ? !out_w : !out_in)) {
+        set_error("winding-number grid: invalid grid or null output");
+        return SSDF_ERR_INVALID;
+    }
+    const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
+    if (h->m.nsel <= 0) return fill_empty(mode, nq, out_w, out_in, stream);
+    QueryArgs a{};
+    if (int rc = prepare_args(h, a, nq, 0, stream)) return rc;
+    setup_grid(h->m, a, nx, ny, nz, lo, hi);
+    a.out_w = out_w;
+    a.out_in = out_in;
+    return launch_gwn(h->m, a, QSRC_GRID, mode, h->m.dtype, stream);
 }
 
 // RAII holder of device buffers for the synchronous host-pointer entry points.
@@ -351,9 +421,128 @@ int host_grid_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, 
     return SSDF_OK;
 }
 
+// points_inside_bvh (serial/bvh/bvh.cpp:318-365): out[i] = 1 inside, 0 outside; synchronous.
+template <typename G>
+int host_inside_points(ptrdiff_t npoints, const G *x, const G *y, const G *z, ptrdiff_t nsel, const int *s0,
+                       const int *s1, const int *s2, ptrdiff_t nsp, const G *sx, const G *sy, const G *sz,
+                       uint8_t *out) {
+    if (npoints <= 0) return SSDF_OK;
+    if (!x || !y || !z || !out) {
+        set_error("points_inside: null point or output array");
+        return SSDF_ERR_INVALID;
+    }
+    if (nsel == 0 || nsp == 0) {
+        std::memset(out, 0, (size_t)npoints);
+        return SSDF_OK;
+    }
+    hipStream_t stream = nullptr;
+    ssdf_mesh *h = nullptr;
+    const int dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
+    struct Guard {
+        ssdf_mesh *h;
+        ~Guard() { destroy_mesh(h); }
+    } guard{h};
+    DevBuf dx, dy, dz, dout;
+    if (int rc = h2d(dx, x, npoints)) return rc;
+    if (int rc = h2d(dy, y, npoints)) return rc;
+    if (int rc = h2d(dz, z, npoints)) return rc;
+    if (int rc = h2d(dout, (const uint8_t *)nullptr, npoints)) return rc;
+    if (int rc = gwn_points(h, GW_INSIDE, npoints, dx.p, dy.p, dz.p, nullptr, (unsigned char *)dout.p, stream))
+        return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    return d2h(out, dout, npoints);
+}
+
+// sample_inside_bvh (serial/bvh/bvh.cpp:1800-1866): grid nodes x_i = min + i h, x fastest.
+template <typename G>
+int host_inside_grid(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp, const G *sx,
+                     const G *sy, const G *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, G xmin, G ymin, G zmin,
+                     G xmax, G ymax, G zmax, uint8_t *out) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return SSDF_OK;
+    if (!out) {
+        set_error("sample_inside: null output array");
+        return SSDF_ERR_INVALID;
+    }
+    const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
+    if (nsel == 0 || nsp == 0) {
+        std::memset(out, 0, nq);
+        return SSDF_OK;
+    }
+    hipStream_t stream = nullptr;
+    ssdf_mesh *h = nullptr;
+    const int dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
+    struct Guard {
+        ssdf_mesh *h;
+        ~Guard() { destroy_mesh(h); }
+    } guard{h};
+    DevBuf dout;
+    if (int rc = h2d(dout, (const uint8_t *)nullptr, (ptrdiff_t)nq)) return rc;
+    const double lo[3] = {(double)xmin, (double)ymin, (double)zmin}, hi[3] = {(double)xmax, (double)ymax, (double)zmax};
+    if (int rc = gwn_grid(h, GW_INSIDE, nx, ny, nz, lo, hi, nullptr, (unsigned char *)dout.p, stream)) return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    return d2h(out, dout, (ptrdiff_t)nq);
+}
+
 }  // namespace
 
 extern "C" {
+
+int ssdf_points_inside_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx,
+                         const float *sy, const float *sz, uint8_t *out) {
+    return host_inside_points<float>(npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz, out);
+}
+
+int ssdf_points_inside_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nselements,
+                         const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const double *sx,
+                         const double *sy, const double *sz, uint8_t *out) {
+    return host_inside_points<double>(npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz, out);
+}
+
+int ssdf_sample_inside_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                         float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, uint8_t *out) {
+    return host_inside_grid<float>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                   ymax, zmax, out);
+}
+
+int ssdf_sample_inside_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                         const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                         ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                         uint8_t *out) {
+    return host_inside_grid<double>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                    ymax, zmax, out);
+}
+
+int ssdf_mesh_winding_number(ssdf_mesh_t mesh, ptrdiff_t npoints, const void *x, const void *y, const void *z,
+                             float *out_w, void *stream) {
+    if (!mesh) {
+        set_error("ssdf_mesh_winding_number: null mesh");
+        return SSDF_ERR_INVALID;
+    }
+    return gwn_points(mesh, GW_WINDING, npoints, x, y, z, out_w, nullptr, (hipStream_t)stream);
+}
+
+int ssdf_mesh_inside_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const void *x, const void *y, const void *z,
+                            uint8_t *out_inside, void *stream) {
+    if (!mesh) {
+        set_error("ssdf_mesh_inside_points: null mesh");
+        return SSDF_ERR_INVALID;
+    }
+    return gwn_points(mesh, GW_INSIDE, npoints, x, y, z, nullptr, out_inside, (hipStream_t)stream);
+}
+
+int ssdf_mesh_inside_grid(ssdf_mesh_t mesh, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin, double ymin,
+                          double zmin, double xmax, double ymax, double zmax, uint8_t *out_inside, void *stream) {
+    if (!mesh) {
+        set_error("ssdf_mesh_inside_grid: null mesh");
+        return SSDF_ERR_INVALID;
+    }
+    const double lo[3] = {xmin, ymin, zmin}, hi[3] = {xmax, ymax, zmax};
+    return gwn_grid(mesh, GW_INSIDE, nx, ny, nz, lo, hi, nullptr, out_inside, (hipStream_t)stream);
+}
 
 int ssdf_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
                const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
@@ -444,6 +633,7 @@ int ssdf_mesh_rebuild(ssdf_mesh_t mesh, int with_normals, void *stream) {
         mesh->m.stream = st;
     }
     mesh->m.has_normals = false;
+    mesh->m.has_gwn = false;
     if (int rc = build_lbvh(mesh->m, st)) return rc;
     if (with_normals)
         if (int rc = build_normals(mesh->m, st)) return rc;

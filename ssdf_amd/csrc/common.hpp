@@ -19,13 +19,25 @@
 
 namespace ssdf_amd {
 
+// Internal LBVH node: both children's FP32 boxes, the child references (>= 0 internal node,
+// ~k leaf k) and the node's range of leaves [first, last] in Morton order.
 struct alignas(16) Node {
     float lmin[3], lmax[3];
     float rmin[3], rmax[3];
     int left, right;
-    int pad0, pad1;
+    int first, last;
 };
 static_assert(sizeof(Node) == 64, "Node must be 64 bytes");
+
+// Far-field data of an internal node's two children for winding numbers: the child subtree's
+// summed area vector sum(a_i n_i), its area-weighted centroid and the squared radius of its box
+// around that centroid; plus the child references (as in Node).
+struct alignas(16) GwnNode {
+    float ln[3], lc[3], lr2;
+    float rn[3], rc[3], rr2;
+    int left, right;
+};
+static_assert(sizeof(GwnNode) == 64, "GwnNode must be 64 bytes");
 
 struct alignas(16) TriF {
     float a[3], b[3], c[3];

@@ -297,8 +297,8 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t *__restrict__ cod
     }
     nodes[i].left = left;
     nodes[i].right = right;
-    nodes[i].pad0 = 0;
-    nodes[i].pad1 = 0;
+    nodes[i].first = lo;
+    nodes[i].last = hi;
 }
 
 template <typename G>
@@ -647,6 +647,7 @@ void free_mesh(DevMesh &m) {
     dev_free(m.face_n, st);
     dev_free(m.edge_n, st);
     dev_free(m.vertex_n, st);
+    dev_free(m.gwn, st);
     dev_free(m.ws, st);
     m = DevMesh{};
 }

@@ -34,6 +34,9 @@ struct DevMesh {
     // pseudonormals (FP64): face 3T, edge 9T, vertex 3V
     double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
     bool has_normals = false;
+    // winding-number far field (nsel - 1 records), built on first use, stale after a rebuild
+    GwnNode *gwn = nullptr;
+    bool has_gwn = false;
     size_t device_bytes = 0;
     hipStream_t stream = nullptr;  // allocations and frees are ordered on this stream
     // build workspace, kept across rebuilds (the LBVH build and the normals build run one after the
@@ -55,6 +58,9 @@ void dev_free(void *ptr, hipStream_t stream);
 int build_lbvh(DevMesh &m, hipStream_t stream);
 // FP64 pseudonormals on the device (face, edge, vertex), following serial/bvh/bvh.cpp:1669-1721.
 int build_normals(DevMesh &m, hipStream_t stream);
+// Far-field tree of the generalized winding number (gwn.hip): per internal node, each child's
+// area vector, centroid and radius, from FP64 prefix sums over the Morton-ordered leaves.
+int build_gwn(DevMesh &m, hipStream_t stream);
 // Copies meta back (synchronises the stream).
 int sync_meta(DevMesh &m, hipStream_t stream);
 void free_mesh(DevMesh &m);

@@ -41,47 +41,6 @@ __device__ __forceinline__ float band_sq(float d2, float tau1, float babs) {
     return r * r;
 }
 
-template <typename T>
-__device__ __forceinline__ T load_as(const void *p, long long i);
-template <>
-__device__ __forceinline__ float load_as<float>(const void *p, long long i) {
-    return ((const float *)p)[i];
-}
-template <>
-__device__ __forceinline__ double load_as<double>(const void *p, long long i) {
-    return ((const double *)p)[i];
-}
-
-// Query of a lane.  Grid: waves map to 4x4x4 bricks, bricks to 4x4x4 super-bricks in Morton order
-// (spatially compact waves and blocks), lanes to (lane&3, lane>>2&3, lane>>4).  Points: 64
-// consecutive points per wave.
-template <int QSRC, typename Q>
-__device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id, int lane, long long &qid, Q &x,
-                                           Q &y, Q &z) {
-    if (QSRC == QSRC_POINTS) {
-        qid = (long long)wave_id * 64 + lane;
-        if (qid >= a.npoints) return false;
-        x = load_as<Q>(a.qx, qid), y = load_as<Q>(a.qy, qid), z = load_as<Q>(a.qz, qid);
-        return true;
-    } else {
-        const unsigned sb = wave_id >> 6, l = wave_id & 63u;
-        const unsigned nsx = (a.nbx + 3) >> 2, nsy = (a.nby + 3) >> 2;
-        const unsigned sbx = sb % nsx, sby = (sb / nsx) % nsy, sbz = sb / (nsx * nsy);
-        const unsigned lx = (l & 1u) | ((l >> 2) & 2u), ly = ((l >> 1) & 1u) | ((l >> 3) & 2u),
-                       lz = ((l >> 2) & 1u) | ((l >> 4) & 2u);
-        const int i = (int)((sbx * 4 + lx) * 4) + (lane & 3);
-        const int j = (int)((sby * 4 + ly) * 4) + ((lane >> 2) & 3);
-        const int k = (int)((sbz * 4 + lz) * 4) + (lane >> 4);
-        if (i >= a.nx || j >= a.ny || k >= a.nz) return false;
-        qid = (long long)i + (long long)a.nx * ((long long)j + (long long)a.ny * k);
-        const Q hx = (Q)a.gh[0], hy = (Q)a.gh[1], hz = (Q)a.gh[2];
-        x = (Q)a.gmin[0] + (Q)i * hx;
-        y = (Q)a.gmin[1] + (Q)j * hy;
-        z = (Q)a.gmin[2] + (Q)k * hz;
-        return true;
-    }
-}
-
 // Sign of (q - c).n at the closest point c of triangle (a, b, c_); region by barycentrics with the
 // reference's tolerance; follows serial/bvh/bvh.cpp:1734-1792.  Returns the dot product and |n|.
 template <typename T>
@@ -127,10 +86,6 @@ __device__ __forceinline__ T sign_dot(const T qx, const T qy, const T qz, const 
     return (qx - c[0]) * n0 + (qy - c[1]) * n1 + (qz - c[2]) * n2;
 }
 
-template <typename T>
-__device__ __forceinline__ void store_as(void *p, long long i, T v) {
-    ((T *)p)[i] = v;
-}
 
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).

@@ -52,9 +52,67 @@ struct QueryArgs {
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
     unsigned nblocks;  // launched blocks (XCD remap)
     unsigned flags;    // QF_* of the call
+    // winding numbers (gwn.hip)
+    const GwnNode *gwn;
+    float beta2;          // far-field acceptance: |c - q|^2 > beta2 * r^2
+    float *out_w;         // GW_WINDING
+    unsigned char *out_in;  // GW_INSIDE
 };
 
 int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,
                  int q_is_double, hipStream_t stream);
+
+// Generalized winding number of the queries (gwn.hip): GW_WINDING writes w (float) to out_w,
+// GW_INSIDE writes w >= 1/2 (uint8) to out_in, GW_SIGN gives out_d (mesh precision) the sign
+// -|d| where w >= 1/2, +|d| elsewhere.  Builds the mesh's dipole tree on first use.
+enum GwnMode : int { GW_WINDING = 0, GW_INSIDE = 1, GW_SIGN = 2 };
+int launch_gwn(DevMesh &m, QueryArgs &a, int qsrc, int mode, int q_is_double, hipStream_t stream);
+
+template <typename T>
+__device__ __forceinline__ T load_as(const void *p, long long i);
+template <>
+__device__ __forceinline__ float load_as<float>(const void *p, long long i) {
+    return ((const float *)p)[i];
+}
+template <>
+__device__ __forceinline__ double load_as<double>(const void *p, long long i) {
+    return ((const double *)p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void store_as(void *p, long long i, T v) {
+    ((T *)p)[i] = v;
+}
+
+// Query of a lane.  Grid: waves map to 4x4x4 bricks, bricks to 4x4x4 super-bricks in Morton order
+// (spatially compact waves and blocks), lanes to (lane&3, lane>>2&3, lane>>4).  Points: 64
+// consecutive points per wave.
+template <int QSRC, typename Q>
+__device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id, int lane, long long &qid, Q &x,
+                                           Q &y, Q &z) {
+    if (QSRC == QSRC_POINTS) {
+        qid = (long long)wave_id * 64 + lane;
+        if (qid >= a.npoints) return false;
+        x = load_as<Q>(a.qx, qid), y = load_as<Q>(a.qy, qid), z = load_as<Q>(a.qz, qid);
+        return true;
+    } else {
+        const unsigned sb = wave_id >> 6, l = wave_id & 63u;
+        const unsigned nsx = (a.nbx + 3) >> 2, nsy = (a.nby + 3) >> 2;
+        const unsigned sbx = sb % nsx, sby = (sb / nsx) % nsy, sbz = sb / (nsx * nsy);
+        const unsigned lx = (l & 1u) | ((l >> 2) & 2u), ly = ((l >> 1) & 1u) | ((l >> 3) & 2u),
+                       lz = ((l >> 2) & 1u) | ((l >> 4) & 2u);
+        const int i = (int)((sbx * 4 + lx) * 4) + (lane & 3);
+        const int j = (int)((sby * 4 + ly) * 4) + ((lane >> 2) & 3);
+        const int k = (int)((sbz * 4 + lz) * 4) + (lane >> 4);
+        if (i >= a.nx || j >= a.ny || k >= a.nz) return false;
+        qid = (long long)i + (long long)a.nx * ((long long)j + (long long)a.ny * k);
+        const Q hx = (Q)a.gh[0], hy = (Q)a.gh[1], hz = (Q)a.gh[2];
+        x = (Q)a.gmin[0] + (Q)i * hx;
+        y = (Q)a.gmin[1] + (Q)j * hy;
+        z = (Q)a.gmin[2] + (Q)k * hz;
+        return true;
+    }
+}
+
 
 }  // namespace ssdf_amd
