@@ -196,9 +196,52 @@ int ssdf_mesh_inside_grid(ssdf_mesh_t mesh, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_
                           double zmin, double xmax, double ymax, double zmax, uint8_t *out_inside, void *stream);
 
 /* Statistics of the last query on this handle (after its stream has completed): number of
- * queries re-traversed in FP64 because their candidate list overflowed, and the kernel error flag
- * (non-zero if a traversal stack overflowed). */
+ * queries re-traversed in FP64 because their candidate list overflowed, and the kernel error flag:
+ * bit 1 a traversal stack overflowed, bit 2 the iteration bound was hit (malformed tree), bit 4
+ * the fixup queue (max(65536, queries / 8) entries) was full and some near-tied queries kept their
+ * FP32 result. */
 int ssdf_mesh_last_stats(ssdf_mesh_t mesh, unsigned *fixup_queued, int *kernel_error);
+
+/* ---------------------------------------------------------------------------------------------
+ * Batches (no reference counterpart: the reference answers one mesh per call, bvh.cu:80-158).
+ * Many FP32 meshes live in shared device arenas; one call answers every mesh's queries with a
+ * single persistent launch (work queue over (mesh, 64-query unit) items, one range per XCD with
+ * stealing).  Meshes are given concatenated: mesh i owns triangles [tri_offsets[i],
+ * tri_offsets[i+1]) and vertices [vert_offsets[i], vert_offsets[i+1]), its s0/s1/s2 holding LOCAL
+ * vertex indices.  Queries and outputs are device arrays.  Calls on one batch must not overlap.
+ * --------------------------------------------------------------------------------------------- */
+typedef struct ssdf_batch *ssdf_batch_t;
+
+int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t *vert_offsets, const int *s0,
+                      const int *s1, const int *s2, const float *sx, const float *sy, const float *sz, int device_ptrs,
+                      int with_normals, void *stream, ssdf_batch_t *out);
+int ssdf_batch_destroy(ssdf_batch_t batch);
+/* Rebuilds every mesh's LBVH (and normals) from the batch's vertex arena, in place. */
+int ssdf_batch_rebuild(ssdf_batch_t batch, int with_normals, void *stream);
+/* Mesh i's grid: dims[3i..3i+2] nodes over boxes[6i..6i+5] (min xyz, max xyz), x fastest, its
+ * results at out_d/out_tri + out_offsets[i].  mode EDF (min-accumulate) or SDF (overwrite). */
+int ssdf_batch_query_grids(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, const double *boxes,
+                           const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream);
+/* Mesh i's points: [point_offsets[i], point_offsets[i+1]) of x/y/z, results at the same positions. */
+int ssdf_batch_query_points(ssdf_batch_t batch, int mode, const ptrdiff_t *point_offsets, const float *x,
+                            const float *y, const float *z, float *out_d, int *out_tri, float *out_cx, float *out_cy,
+                            float *out_cz, unsigned flags, void *stream);
+/* Persistent-launch schedule: blocks per CU (grid size) and work units per queue grab; 0 = default. */
+int ssdf_batch_set_schedule(ssdf_batch_t batch, int blocks_per_cu, int units_per_grab);
+/* Times the grid query over a small schedule space (blocks per CU x units per grab), keeps the
+ * fastest and reports it.  Runs the query 40 times. */
+int ssdf_batch_autotune(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, const double *boxes,
+                        const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
+                        int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms);
+/* Captures [rebuild of every mesh +] the grid query into a hipGraph (stream must be a created
+ * stream); ssdf_batch_replay launches it.  The graph holds kernel launches only. */
+int ssdf_batch_capture(ssdf_batch_t batch, int rebuild, int mode, const ptrdiff_t *dims, const double *boxes,
+                       const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream);
+int ssdf_batch_replay(ssdf_batch_t batch, void *stream);
+/* As ssdf_mesh_last_stats, for the batch's last query (synchronous read). */
+int ssdf_batch_last_stats(ssdf_batch_t batch, unsigned *fixup_queued, int *kernel_error);
+/* mesh = -1: totals of the batch (triangles, vertices, deepest tree, device bytes). */
+int ssdf_batch_info(ssdf_batch_t batch, int mesh, ssdf_mesh_info_t *info);
 
 /* ---------------------------------------------------------------------------------------------
  * Device plumbing for callers without a GPU runtime of their own (tests, benchmarks, FFI).

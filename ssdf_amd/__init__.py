@@ -23,7 +23,7 @@ __all__ = [
     "sample_sdf_bvh", "sdf", "ResidentMesh", "DeviceArray", "Stream", "Event", "FLAG_FIXUP", "FLAG_SIGN_FP64",
     "FLAG_SIGN_GWN", "FLAG_SIGN_AUTO", "points_inside_bvh", "sample_inside_bvh", "compute_aabb",
     "all_points_aabb_signed_distance", "init_volume_box",
-    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST",
+    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST", "Batch",
 ]
 
 FLAG_FIXUP = 1
@@ -83,6 +83,18 @@ _SIGS = {
                              C.c_double, C.c_double, _P, _P, C.c_uint, _P],
     "ssdf_mesh_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
     "ssdf_mesh_brute_points": [_P, _PD, _P, _P, _P, _P, _P, _P],
+    "ssdf_batch_create": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
+    "ssdf_batch_destroy": [_P],
+    "ssdf_batch_rebuild": [_P, C.c_int, _P],
+    "ssdf_batch_query_grids": [_P, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P],
+    "ssdf_batch_query_points": [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
+    "ssdf_batch_set_schedule": [_P, C.c_int, C.c_int],
+    "ssdf_batch_autotune": [_P, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                            C.POINTER(C.c_float)],
+    "ssdf_batch_capture": [_P, C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P],
+    "ssdf_batch_replay": [_P, _P],
+    "ssdf_batch_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
+    "ssdf_batch_info": [_P, C.c_int, C.POINTER(_MeshInfo)],
     "ssdf_device_count": [C.POINTER(C.c_int)],
     "ssdf_device_malloc": [C.POINTER(_P), C.c_size_t],
     "ssdf_device_free": [_P],
@@ -458,6 +470,110 @@ class ResidentMesh:
     def close(self):
         if getattr(self, "h", None):
             lib().ssdf_mesh_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------------------------
+# batches (device API)
+# ------------------------------------------------------------------------------------------------
+class Batch:
+    """Many FP32 meshes resident in shared arenas, answered by one persistent launch per call."""
+
+    def __init__(self, meshes, with_normals: bool = True, stream: Stream | None = None):
+        tri_off = np.zeros(len(meshes) + 1, np.int64)
+        vert_off = np.zeros(len(meshes) + 1, np.int64)
+        for i, m in enumerate(meshes):
+            tri_off[i + 1] = tri_off[i] + m.s0.size
+            vert_off[i + 1] = vert_off[i] + m.sx.size
+        cat = lambda k, dt: np.ascontiguousarray(np.concatenate([getattr(m, k) for m in meshes]), dtype=dt)
+        s0, s1, s2 = cat("s0", np.int32), cat("s1", np.int32), cat("s2", np.int32)
+        sx, sy, sz = cat("sx", np.float32), cat("sy", np.float32), cat("sz", np.float32)
+        self.nmeshes = len(meshes)
+        self.ntris = tri_off[1:] - tri_off[:-1]
+        h = C.c_void_p()
+        rc = lib().ssdf_batch_create(len(meshes), _ptr(tri_off), _ptr(vert_off), _ptr(s0), _ptr(s1), _ptr(s2),
+                                     _ptr(sx), _ptr(sy), _ptr(sz), 0, 1 if with_normals else 0,
+                                     C.c_void_p(stream.ptr) if stream else None, C.byref(h))
+        _check(rc, "ssdf_batch_create")
+        self.h = h.value
+
+    @staticmethod
+    def grid_layout(dims):
+        """out_offsets for per-mesh grids dims[i] = (nx, ny, nz), packed one after another."""
+        n = np.array([int(d[0]) * int(d[1]) * int(d[2]) for d in dims], np.int64)
+        off = np.zeros(len(dims), np.int64)
+        off[1:] = np.cumsum(n)[:-1]
+        return off, int(n.sum())
+
+    def query_grids(self, mode, dims, boxes, out_offsets, out_d, out_tri=None, flags=0, stream: Stream | None = None):
+        d = np.ascontiguousarray(dims, np.int64).reshape(-1)
+        bx = np.ascontiguousarray(boxes, np.float64).reshape(-1)
+        off = np.ascontiguousarray(out_offsets, np.int64)
+        rc = lib().ssdf_batch_query_grids(C.c_void_p(self.h), mode, _ptr(d), _ptr(bx), _ptr(off), _ptr(out_d),
+                                          _ptr(out_tri), flags, C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_batch_query_grids")
+
+    def query_points(self, mode, point_offsets, x, y, z, out_d, out_tri=None, cx=None, cy=None, cz=None, flags=0,
+                     stream: Stream | None = None):
+        off = np.ascontiguousarray(point_offsets, np.int64)
+        rc = lib().ssdf_batch_query_points(C.c_void_p(self.h), mode, _ptr(off), _ptr(x), _ptr(y), _ptr(z),
+                                           _ptr(out_d), _ptr(out_tri), _ptr(cx), _ptr(cy), _ptr(cz), flags,
+                                           C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_batch_query_points")
+
+    def rebuild(self, with_normals: bool = True, stream: Stream | None = None):
+        _check(lib().ssdf_batch_rebuild(C.c_void_p(self.h), 1 if with_normals else 0,
+                                        C.c_void_p(stream.ptr) if stream else None), "ssdf_batch_rebuild")
+
+    def set_schedule(self, blocks_per_cu: int, units_per_grab: int):
+        _check(lib().ssdf_batch_set_schedule(C.c_void_p(self.h), blocks_per_cu, units_per_grab),
+               "ssdf_batch_set_schedule")
+
+    def autotune(self, mode, dims, boxes, out_offsets, out_d, out_tri=None, flags=0, stream: Stream | None = None):
+        d = np.ascontiguousarray(dims, np.int64).reshape(-1)
+        bx = np.ascontiguousarray(boxes, np.float64).reshape(-1)
+        off = np.ascontiguousarray(out_offsets, np.int64)
+        bb, bc, ms = C.c_int(), C.c_int(), C.c_float()
+        rc = lib().ssdf_batch_autotune(C.c_void_p(self.h), mode, _ptr(d), _ptr(bx), _ptr(off), _ptr(out_d),
+                                       _ptr(out_tri), flags, C.c_void_p(stream.ptr) if stream else None, C.byref(bb),
+                                       C.byref(bc), C.byref(ms))
+        _check(rc, "ssdf_batch_autotune")
+        return int(bb.value), int(bc.value), float(ms.value)
+
+    def capture(self, rebuild, mode, dims, boxes, out_offsets, out_d, out_tri=None, flags=0,
+                stream: Stream | None = None):
+        d = np.ascontiguousarray(dims, np.int64).reshape(-1)
+        bx = np.ascontiguousarray(boxes, np.float64).reshape(-1)
+        off = np.ascontiguousarray(out_offsets, np.int64)
+        self._keep = (d, bx, off)
+        rc = lib().ssdf_batch_capture(C.c_void_p(self.h), 1 if rebuild else 0, mode, _ptr(d), _ptr(bx), _ptr(off),
+                                      _ptr(out_d), _ptr(out_tri), flags, C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_batch_capture")
+
+    def replay(self, stream: Stream | None = None):
+        _check(lib().ssdf_batch_replay(C.c_void_p(self.h), C.c_void_p(stream.ptr) if stream else None),
+               "ssdf_batch_replay")
+
+    def last_stats(self):
+        q, e = C.c_uint(), C.c_int()
+        _check(lib().ssdf_batch_last_stats(C.c_void_p(self.h), C.byref(q), C.byref(e)), "ssdf_batch_last_stats")
+        return int(q.value), int(e.value)
+
+    def info(self, mesh: int = -1) -> dict:
+        i = _MeshInfo()
+        _check(lib().ssdf_batch_info(C.c_void_p(self.h), mesh, C.byref(i)), "ssdf_batch_info")
+        return dict(nselements=i.nselements, nspoints=i.nspoints, maxdepth=i.maxdepth, open_edges=i.open_edges,
+                    flipped_edges=i.flipped_edges, bbox=list(i.bbox), device_bytes=i.device_bytes)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ssdf_batch_destroy(C.c_void_p(self.h))
             self.h = None
 
     def __del__(self):

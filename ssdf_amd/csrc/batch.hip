@@ -1,0 +1,590 @@
+// batch.hip — many FP32 meshes resident in shared arenas, queried by one persistent launch.
+//
+// Every mesh keeps the single-mesh layout (LBVH nodes, Morton-ordered TriP records, pseudonormals,
+// meta record), placed at its offset in one arena per kind.  The kernels take the arena bases as
+// __restrict__ arguments and a wave adds its mesh's (uniform) offset, so node records still arrive
+// by scalar loads.  A query call is three launches on the caller's stream — reset, the persistent
+// packet kernel over all meshes' work units, the fixup-queue kernel — so a build + query sequence
+// can be captured once into a hipGraph and replayed.
+//
+// Work order: units are numbered mesh after mesh, meshes sorted by decreasing tree depth (the
+// costlier queries first), and split into 8 ranges, one per XCD, with stealing (k_batch_packet).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "query.hpp"
+
+using namespace ssdf_amd;
+
+struct ssdf_batch {
+    int nm = 0;
+    bool with_normals = false;
+    hipStream_t stream = nullptr;  // allocations are ordered on the creation stream
+    std::vector<DevMesh> meshes;   // views into the arenas
+    std::vector<long long> tri_off, vert_off, node_off;
+    int *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+    float *sx = nullptr, *sy = nullptr, *sz = nullptr;
+    Node *nodes = nullptr;
+    TriP *tris = nullptr;
+    MeshMeta *metas = nullptr;
+    double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
+    char *ws = nullptr;
+    size_t ws_cap = 0;
+    size_t device_bytes = 0;
+    // query state
+    BatchMesh *d_desc = nullptr, *h_desc = nullptr;  // device / pinned host descriptors
+    int desc_cap = 0;
+    unsigned *queue = nullptr;   // 8 x 64 words
+    unsigned *status = nullptr;  // fixup count, error flag
+    unsigned *fix_list = nullptr;
+    size_t fix_cap = 0;  // entries
+    int maxdepth = 0;
+    int num_cus = 256;
+    int blocks_per_cu = 0, chunk = 0;  // schedule (0: defaults)
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+};
+
+namespace {
+
+void destroy_batch(ssdf_batch *b) {
+    if (!b) return;
+    const hipStream_t st = b->stream;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+    (void)hipStreamSynchronize(st);
+    for (void *p : {(void *)b->s0, (void *)b->s1, (void *)b->s2, (void *)b->sx, (void *)b->sy, (void *)b->sz,
+                    (void *)b->nodes, (void *)b->tris, (void *)b->metas, (void *)b->face_n, (void *)b->edge_n,
+                    (void *)b->vertex_n, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
+                    (void *)b->fix_list})
+        dev_free(p, st);
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    (void)hipStreamSynchronize(st);
+    delete b;
+}
+
+template <typename T>
+int arena(T **p, size_t n, ssdf_batch *b) {
+    SSDF_HIP_TRY(dev_alloc((void **)p, sizeof(T) * std::max<size_t>(n, 1), b->stream));
+    b->device_bytes += sizeof(T) * n;
+    return SSDF_OK;
+}
+
+// Builds mesh i (LBVH, and normals) with the batch's shared workspace.
+int build_one(ssdf_batch *b, int i, bool normals, hipStream_t st) {
+    DevMesh &m = b->meshes[i];
+    m.ws = b->ws, m.ws_cap = b->ws_cap;
+    int rc = build_lbvh(m, st);
+    if (!rc && normals) rc = build_normals(m, st);
+    b->ws = m.ws, b->ws_cap = m.ws_cap;  // grown if it had to
+    m.ws = nullptr, m.ws_cap = 0;
+    return rc;
+}
+
+// Copies every meta record back (one synchronisation) and records the deepest tree.
+int sync_metas(ssdf_batch *b, hipStream_t st) {
+    std::vector<MeshMeta> h(b->nm);
+    SSDF_HIP_TRY(hipMemcpyAsync(h.data(), b->metas, sizeof(MeshMeta) * b->nm, hipMemcpyDeviceToHost, st));
+    SSDF_HIP_TRY(hipStreamSynchronize(st));
+    b->maxdepth = 0;
+    for (int i = 0; i < b->nm; i++) {
+        if (b->meshes[i].nsel <= 0) {  // never built: no record on the device
+            b->meshes[i].hmeta = MeshMeta{};
+            continue;
+        }
+        b->meshes[i].hmeta = h[i];
+        b->maxdepth = std::max(b->maxdepth, h[i].maxdepth);
+    }
+    return SSDF_OK;
+}
+
+int rebuild_all(ssdf_batch *b, bool normals, hipStream_t st) {
+    // the largest mesh first, so that the shared workspace grows once
+    std::vector<int> order(b->nm);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->meshes[x].nsel > b->meshes[y].nsel; });
+    for (int i : order)
+        if (b->meshes[i].nsel > 0)
+            if (int rc = build_one(b, i, normals, st)) return rc;
+    for (int i = 0; i < b->nm; i++) b->meshes[i].has_normals = normals && b->meshes[i].nsel > 0;
+    return SSDF_OK;
+}
+
+int ensure_desc(ssdf_batch *b, int n) {
+    if (b->desc_cap >= n) return SSDF_OK;
+    dev_free(b->d_desc, b->stream);
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    b->d_desc = nullptr, b->h_desc = nullptr, b->desc_cap = 0;
+    SSDF_HIP_TRY(dev_alloc((void **)&b->d_desc, sizeof(BatchMesh) * n, b->stream));
+    SSDF_HIP_TRY(hipHostMalloc((void **)&b->h_desc, sizeof(BatchMesh) * n, hipHostMallocDefault));
+    b->desc_cap = n;
+    return SSDF_OK;
+}
+
+int ensure_fix(ssdf_batch *b, size_t entries) {
+    if (b->fix_cap >= entries) return SSDF_OK;
+    dev_free(b->fix_list, b->stream);
+    b->fix_list = nullptr, b->fix_cap = 0;
+    SSDF_HIP_TRY(dev_alloc((void **)&b->fix_list, sizeof(unsigned) * 4 * entries, b->stream));
+    b->fix_cap = entries;
+    return SSDF_OK;
+}
+
+// Per-mesh descriptors of a query (grid: dims/boxes, points: offsets) in work-unit order.
+struct Plan {
+    std::vector<BatchMesh> desc;
+    unsigned total_units = 0;
+    size_t total_queries = 0;
+};
+
+int plan_queries(ssdf_batch *b, int qsrc, const ptrdiff_t *dims, const double *boxes, const ptrdiff_t *offsets,
+                 Plan &p) {
+    std::vector<int> order(b->nm);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return b->meshes[x].hmeta.maxdepth > b->meshes[y].hmeta.maxdepth;
+    });
+    unsigned long long units = 0;
+    for (int i : order) {
+        const DevMesh &m = b->meshes[i];
+        BatchMesh d{};
+        d.node_off = b->node_off[i], d.tri_off = b->tri_off[i], d.vert_off = b->vert_off[i];
+        d.mesh = i;
+        const float L = meta_scale(m.hmeta);
+        d.band_abs = kBandRel * L;
+        d.sign_eps = (float)(kSignEpsRel * meta_diag(m.hmeta));
+        d.delta64 = kDelta64Rel * L;
+        unsigned long long u = 0;
+        if (qsrc == QSRC_GRID) {
+            const ptrdiff_t n[3] = {dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]};
+            if (n[0] < 0 || n[1] < 0 || n[2] < 0 || n[0] > 0x7fffffff || n[1] > 0x7fffffff || n[2] > 0x7fffffff) {
+                set_error("batch grid: invalid dimensions");
+                return SSDF_ERR_INVALID;
+            }
+            d.nx = (int)n[0], d.ny = (int)n[1], d.nz = (int)n[2];
+            d.out_off = offsets[i];
+            const double *lo = boxes + 6 * i, *hi = boxes + 6 * i + 3;
+            for (int k = 0; k < 3; k++) {
+                const float l = (float)lo[k], h = (float)hi[k];
+                d.gmin[k] = l;
+                d.gh[k] = n[k] > 1 ? (double)((h - l) / (float)(n[k] - 1)) : 0.0;
+            }
+            d.nbx = (d.nx + 3) / 4, d.nby = (d.ny + 3) / 4, d.nbz = (d.nz + 3) / 4;
+            const unsigned long long ns = (unsigned long long)((d.nbx + 3) / 4) * ((d.nby + 3) / 4) * ((d.nbz + 3) / 4);
+            if (m.nsel > 0 && d.nx > 0 && d.ny > 0 && d.nz > 0) u = ns * 64;
+            p.total_queries += (size_t)n[0] * n[1] * n[2];
+        } else {
+            d.out_off = offsets[i];
+            d.npoints = offsets[i + 1] - offsets[i];
+            if (d.npoints < 0) {
+                set_error("batch points: offsets must not decrease");
+                return SSDF_ERR_INVALID;
+            }
+            if (m.nsel > 0) u = (unsigned long long)(d.npoints + 63) / 64;
+            p.total_queries += (size_t)d.npoints;
+        }
+        d.unit_start = (unsigned)units;
+        units += u;
+        if (units > 0xffffffffull) {
+            set_error("batch: more than 2^32 work units");
+            return SSDF_ERR_INVALID;
+        }
+        p.desc.push_back(d);
+    }
+    p.total_units = (unsigned)units;
+    return SSDF_OK;
+}
+
+// Uploads the plan (pinned staging, stream-ordered) and fills the launch arguments.
+int stage(ssdf_batch *b, const Plan &p, int qsrc, int mode, unsigned flags, const float *x, const float *y,
+          const float *z, float *out_d, int *out_tri, float *cx, float *cy, float *cz, hipStream_t st,
+          BatchArgs &a) {
+    if (int rc = ensure_desc(b, b->nm)) return rc;
+    if (!b->queue) SSDF_HIP_TRY(dev_alloc((void **)&b->queue, sizeof(unsigned) * 8 * 64, b->stream));
+    if (!b->status) SSDF_HIP_TRY(dev_alloc((void **)&b->status, 64, b->stream));
+    const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
+    const size_t entries = fix ? std::min(p.total_queries, std::max<size_t>(1u << 16, p.total_queries / 8)) : 1;
+    if (int rc = ensure_fix(b, std::max<size_t>(entries, 1))) return rc;
+    if (st != b->stream) {
+        // the buffers above are ordered on the creation stream
+        hipEvent_t ev;
+        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SSDF_HIP_TRY(hipEventRecord(ev, b->stream));
+        SSDF_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
+        (void)hipEventDestroy(ev);
+    }
+    // the pinned staging buffer may still feed an earlier call's copy
+    SSDF_HIP_TRY(hipStreamSynchronize(st));
+    std::memcpy(b->h_desc, p.desc.data(), sizeof(BatchMesh) * p.desc.size());
+    SSDF_HIP_TRY(hipMemcpyAsync(b->d_desc, b->h_desc, sizeof(BatchMesh) * p.desc.size(), hipMemcpyHostToDevice, st));
+    std::memset(&a, 0, sizeof(a));
+    a.base.tau = kTau;
+    a.base.flags = flags;
+    a.base.fix_count = b->status;
+    a.base.err_flag = (int *)(b->status + 1);
+    a.base.fix_list = b->fix_list;
+    a.base.fix_cap = (unsigned)std::min<size_t>(b->fix_cap, 0xffffffffu);
+    long long maxsel = 1;
+    for (auto &m : b->meshes) maxsel = std::max<long long>(maxsel, m.nsel);
+    a.base.max_iters = 4 * maxsel + 64;
+    a.nodes = b->nodes, a.tris = b->tris, a.metas = b->metas;
+    a.s0 = b->s0, a.s1 = b->s1, a.s2 = b->s2;
+    a.sx = b->sx, a.sy = b->sy, a.sz = b->sz;
+    a.face_n = b->face_n, a.edge_n = b->edge_n, a.vertex_n = b->vertex_n;
+    a.qx = x, a.qy = y, a.qz = z;
+    a.out_d = out_d, a.out_tri = out_tri, a.out_cx = cx, a.out_cy = cy, a.out_cz = cz;
+    a.nmeshes = (int)p.desc.size();
+    a.total_units = p.total_units;
+    a.queue = b->queue;
+    a.chunk = (unsigned)(b->chunk > 0 ? b->chunk : 2);
+    (void)qsrc;
+    (void)mode;
+    return SSDF_OK;
+}
+
+int stack_of(const ssdf_batch *b) {
+    if (b->maxdepth <= 64) return 64;
+    if (b->maxdepth <= 128) return 128;
+    return 0;
+}
+
+unsigned grid_blocks(const ssdf_batch *b, unsigned total_units, bool fix) {
+    const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu : (fix ? 5 : 7);
+    const unsigned want = (unsigned)(b->num_cus * bpc);
+    return std::max(1u, std::min(want, (total_units + 3) / 4));
+}
+
+int run_query(ssdf_batch *b, int qsrc, int mode, const ptrdiff_t *dims, const double *boxes,
+              const ptrdiff_t *offsets, const float *x, const float *y, const float *z, float *out_d, int *out_tri,
+              float *cx, float *cy, float *cz, unsigned flags, hipStream_t st) {
+    if (mode == SSDF_MODE_SDF && !b->with_normals) {
+        set_error("batch: signed queries need the meshes' pseudonormals");
+        return SSDF_ERR_INVALID;
+    }
+    if (!out_d || (mode == SSDF_MODE_CLOSEST && (!cx || !cy || !cz))) {
+        set_error("batch query: null output");
+        return SSDF_ERR_INVALID;
+    }
+    Plan p;
+    if (int rc = plan_queries(b, qsrc, dims, boxes, offsets, p)) return rc;
+    if (p.total_units == 0) return SSDF_OK;
+    BatchArgs a;
+    if (int rc = stage(b, p, qsrc, mode, flags, x, y, z, out_d, out_tri, cx, cy, cz, st, a)) return rc;
+    const int stack = stack_of(b);
+    if (!stack) {
+        set_error("batch: LBVH deeper than 128 levels");
+        return SSDF_ERR_DEPTH;
+    }
+    const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
+    return launch_batch(a, b->d_desc, qsrc, mode, fix, stack, grid_blocks(b, p.total_units, fix), st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t *vert_offsets, const int *s0,
+                      const int *s1, const int *s2, const float *sx, const float *sy, const float *sz, int device_ptrs,
+                      int with_normals, void *stream, ssdf_batch_t *out) {
+    if (!out || nmeshes <= 0 || !tri_offsets || !vert_offsets) {
+        set_error("ssdf_batch_create: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    *out = nullptr;
+    const ptrdiff_t T = tri_offsets[nmeshes], V = vert_offsets[nmeshes];
+    if (tri_offsets[0] != 0 || vert_offsets[0] != 0 || T < 0 || V < 0 ||
+        (T > 0 && (!s0 || !s1 || !s2)) || (V > 0 && (!sx || !sy || !sz))) {
+        set_error("ssdf_batch_create: offsets must start at 0; arrays must be non-null");
+        return SSDF_ERR_INVALID;
+    }
+    for (int i = 0; i < nmeshes; i++) {
+        const ptrdiff_t t = tri_offsets[i + 1] - tri_offsets[i], v = vert_offsets[i + 1] - vert_offsets[i];
+        if (t < 0 || v < 0 || t > 0x3fffffff || (t > 0 && v == 0)) {
+            set_error("ssdf_batch_create: offsets must not decrease; a mesh with triangles needs vertices");
+            return SSDF_ERR_INVALID;
+        }
+    }
+    ssdf_batch *b = new (std::nothrow) ssdf_batch();
+    if (!b) return SSDF_ERR_OOM;
+    struct Guard {
+        ssdf_batch *b;
+        ~Guard() {
+            if (b) destroy_batch(b);
+        }
+    } guard{b};
+    b->nm = nmeshes;
+    b->stream = (hipStream_t)stream;
+    b->with_normals = with_normals != 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            b->num_cus = cus;
+    }
+    b->tri_off.resize(nmeshes);
+    b->vert_off.resize(nmeshes);
+    b->node_off.resize(nmeshes);
+    long long nodes_total = 0;
+    for (int i = 0; i < nmeshes; i++) {
+        b->tri_off[i] = tri_offsets[i];
+        b->vert_off[i] = vert_offsets[i];
+        b->node_off[i] = nodes_total;
+        const long long t = tri_offsets[i + 1] - tri_offsets[i];
+        nodes_total += t > 1 ? t - 1 : 1;
+    }
+    if (int rc = arena(&b->s0, T, b)) return rc;
+    if (int rc = arena(&b->s1, T, b)) return rc;
+    if (int rc = arena(&b->s2, T, b)) return rc;
+    if (int rc = arena(&b->sx, V, b)) return rc;
+    if (int rc = arena(&b->sy, V, b)) return rc;
+    if (int rc = arena(&b->sz, V, b)) return rc;
+    if (int rc = arena(&b->nodes, nodes_total, b)) return rc;
+    if (int rc = arena(&b->tris, T, b)) return rc;
+    if (int rc = arena(&b->metas, nmeshes, b)) return rc;
+    if (b->with_normals) {
+        if (int rc = arena(&b->face_n, 3 * (size_t)T, b)) return rc;
+        if (int rc = arena(&b->edge_n, 9 * (size_t)T, b)) return rc;
+        if (int rc = arena(&b->vertex_n, 3 * (size_t)V, b)) return rc;
+    }
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipStream_t st = b->stream;
+    if (T > 0) {
+        SSDF_HIP_TRY(hipMemcpyAsync(b->s0, s0, sizeof(int) * T, kind, st));
+        SSDF_HIP_TRY(hipMemcpyAsync(b->s1, s1, sizeof(int) * T, kind, st));
+        SSDF_HIP_TRY(hipMemcpyAsync(b->s2, s2, sizeof(int) * T, kind, st));
+    }
+    if (V > 0) {
+        SSDF_HIP_TRY(hipMemcpyAsync(b->sx, sx, sizeof(float) * V, kind, st));
+        SSDF_HIP_TRY(hipMemcpyAsync(b->sy, sy, sizeof(float) * V, kind, st));
+        SSDF_HIP_TRY(hipMemcpyAsync(b->sz, sz, sizeof(float) * V, kind, st));
+    }
+    b->meshes.resize(nmeshes);
+    for (int i = 0; i < nmeshes; i++) {
+        DevMesh &m = b->meshes[i];
+        m.dtype = 0;
+        m.nsel = tri_offsets[i + 1] - tri_offsets[i];
+        m.nsp = vert_offsets[i + 1] - vert_offsets[i];
+        m.s0 = b->s0 + b->tri_off[i], m.s1 = b->s1 + b->tri_off[i], m.s2 = b->s2 + b->tri_off[i];
+        m.sx = b->sx + b->vert_off[i], m.sy = b->sy + b->vert_off[i], m.sz = b->sz + b->vert_off[i];
+        m.owns_input = false;
+        m.nodes = b->nodes + b->node_off[i];
+        m.tris = b->tris + b->tri_off[i];
+        m.meta = b->metas + i;
+        if (b->with_normals) {
+            m.face_n = b->face_n + 3 * b->tri_off[i];
+            m.edge_n = b->edge_n + 9 * b->tri_off[i];
+            m.vertex_n = b->vertex_n + 3 * b->vert_off[i];
+        }
+        m.stream = st;
+    }
+    if (int rc = rebuild_all(b, b->with_normals, st)) return rc;
+    if (int rc = sync_metas(b, st)) return rc;
+    b->device_bytes += b->ws_cap;
+    *out = b;
+    guard.b = nullptr;
+    return SSDF_OK;
+}
+
+int ssdf_batch_destroy(ssdf_batch_t batch) {
+    destroy_batch(batch);
+    return SSDF_OK;
+}
+
+int ssdf_batch_rebuild(ssdf_batch_t b, int with_normals, void *stream) {
+    if (!b) {
+        set_error("ssdf_batch_rebuild: null batch");
+        return SSDF_ERR_INVALID;
+    }
+    if (with_normals && !b->with_normals) {
+        set_error("ssdf_batch_rebuild: the batch was created without normals");
+        return SSDF_ERR_INVALID;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    if (st != b->stream) SSDF_HIP_TRY(hipStreamSynchronize(b->stream));
+    if (int rc = rebuild_all(b, with_normals != 0, st)) return rc;
+    return sync_metas(b, st);
+}
+
+int ssdf_batch_query_grids(ssdf_batch_t b, int mode, const ptrdiff_t *dims, const double *boxes,
+                           const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream) {
+    if (!b || !dims || !boxes || !out_offsets || mode == SSDF_MODE_CLOSEST) {
+        set_error("ssdf_batch_query_grids: invalid arguments (closest points need point queries)");
+        return SSDF_ERR_INVALID;
+    }
+    return run_query(b, QSRC_GRID, mode, dims, boxes, out_offsets, nullptr, nullptr, nullptr, out_d, out_tri,
+                     nullptr, nullptr, nullptr, flags, (hipStream_t)stream);
+}
+
+int ssdf_batch_query_points(ssdf_batch_t b, int mode, const ptrdiff_t *point_offsets, const float *x,
+                            const float *y, const float *z, float *out_d, int *out_tri, float *out_cx, float *out_cy,
+                            float *out_cz, unsigned flags, void *stream) {
+    if (!b || !point_offsets || !x || !y || !z || point_offsets[0] != 0) {
+        set_error("ssdf_batch_query_points: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    return run_query(b, QSRC_POINTS, mode, nullptr, nullptr, point_offsets, x, y, z, out_d, out_tri, out_cx, out_cy,
+                     out_cz, flags, (hipStream_t)stream);
+}
+
+int ssdf_batch_set_schedule(ssdf_batch_t b, int blocks_per_cu, int units_per_grab) {
+    if (!b || blocks_per_cu < 0 || blocks_per_cu > 32 || units_per_grab < 0 || units_per_grab > 256) {
+        set_error("ssdf_batch_set_schedule: blocks_per_cu in [0, 32], units_per_grab in [0, 256]");
+        return SSDF_ERR_INVALID;
+    }
+    b->blocks_per_cu = blocks_per_cu;
+    b->chunk = units_per_grab;
+    return SSDF_OK;
+}
+
+int ssdf_batch_autotune(ssdf_batch_t b, int mode, const ptrdiff_t *dims, const double *boxes,
+                        const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
+                        int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms) {
+    if (!b) {
+        set_error("ssdf_batch_autotune: null batch");
+        return SSDF_ERR_INVALID;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    const int bpcs[] = {3, 4, 5, 6, 8};
+    const int chunks[] = {1, 2, 4, 8};
+    hipEvent_t e0, e1;
+    SSDF_HIP_TRY(hipEventCreate(&e0));
+    SSDF_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bb = 0, bc = 0;
+    int rc = SSDF_OK;
+    for (int bpc : bpcs) {
+        for (int ch : chunks) {
+            b->blocks_per_cu = bpc, b->chunk = ch;
+            // one untimed run, then the timed one
+            for (int rep = 0; rep < 2 && !rc; rep++) {
+                if (rep == 1) (void)hipEventRecord(e0, st);
+                rc = ssdf_batch_query_grids(b, mode, dims, boxes, out_offsets, out_d, out_tri, flags, stream);
+                if (rep == 1) (void)hipEventRecord(e1, st);
+            }
+            if (rc) break;
+            float ms = 0.0f;
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) best = ms, bb = bpc, bc = ch;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->blocks_per_cu = bb, b->chunk = bc;
+    if (best_blocks_per_cu) *best_blocks_per_cu = bb;
+    if (best_units_per_grab) *best_units_per_grab = bc;
+    if (best_ms) *best_ms = best;
+    return SSDF_OK;
+}
+
+int ssdf_batch_capture(ssdf_batch_t b, int rebuild, int mode, const ptrdiff_t *dims, const double *boxes,
+                       const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream) {
+    if (!b || !dims || !boxes || !out_offsets || mode == SSDF_MODE_CLOSEST) {
+        set_error("ssdf_batch_capture: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    if (!st) {
+        set_error("ssdf_batch_capture: needs a created stream (the null stream cannot be captured)");
+        return SSDF_ERR_INVALID;
+    }
+    if (mode == SSDF_MODE_SDF && !b->with_normals) {
+        set_error("batch: signed queries need the meshes' pseudonormals");
+        return SSDF_ERR_INVALID;
+    }
+    Plan p;
+    if (int rc = plan_queries(b, QSRC_GRID, dims, boxes, out_offsets, p)) return rc;
+    BatchArgs a;
+    if (int rc = stage(b, p, QSRC_GRID, mode, flags, nullptr, nullptr, nullptr, out_d, out_tri, nullptr, nullptr,
+                       nullptr, st, a))
+        return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(st));  // descriptors and buffers in place before capture
+    const int stack = stack_of(b);
+    if (!stack) {
+        set_error("batch: LBVH deeper than 128 levels");
+        return SSDF_ERR_DEPTH;
+    }
+    if (b->graph) (void)hipGraphExecDestroy(b->graph), b->graph = nullptr;
+    if (b->graph_def) (void)hipGraphDestroy(b->graph_def), b->graph_def = nullptr;
+    const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
+    SSDF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = SSDF_OK;
+    if (rebuild) {
+        std::vector<int> order(b->nm);
+        std::iota(order.begin(), order.end(), 0);
+        for (int i : order)
+            if (!rc && b->meshes[i].nsel > 0) {
+                DevMesh &m = b->meshes[i];
+                m.ws = b->ws, m.ws_cap = b->ws_cap;  // large enough since creation: no allocation
+                rc = build_lbvh(m, st);
+                if (!rc && b->with_normals) rc = build_normals(m, st);
+                m.ws = nullptr, m.ws_cap = 0;
+            }
+    }
+    if (!rc && p.total_units > 0) rc = launch_batch(a, b->d_desc, QSRC_GRID, mode, fix, stack,
+                                                    grid_blocks(b, p.total_units, fix), st);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    SSDF_HIP_TRY(ce);
+    b->graph_def = g;
+    SSDF_HIP_TRY(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+    return SSDF_OK;
+}
+
+int ssdf_batch_replay(ssdf_batch_t b, void *stream) {
+    if (!b || !b->graph) {
+        set_error("ssdf_batch_replay: nothing captured");
+        return SSDF_ERR_INVALID;
+    }
+    SSDF_HIP_TRY(hipGraphLaunch(b->graph, (hipStream_t)stream));
+    return SSDF_OK;
+}
+
+int ssdf_batch_last_stats(ssdf_batch_t b, unsigned *fixup_queued, int *kernel_error) {
+    if (!b) {
+        set_error("ssdf_batch_last_stats: null batch");
+        return SSDF_ERR_INVALID;
+    }
+    unsigned h[2] = {0u, 0u};
+    if (b->status) SSDF_HIP_TRY(hipMemcpy(h, b->status, sizeof(h), hipMemcpyDeviceToHost));
+    if (fixup_queued) *fixup_queued = h[0];
+    if (kernel_error) *kernel_error = (int)h[1];
+    return SSDF_OK;
+}
+
+int ssdf_batch_info(ssdf_batch_t b, int mesh, ssdf_mesh_info_t *info) {
+    if (!b || !info || mesh < -1 || mesh >= b->nm) {
+        set_error("ssdf_batch_info: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    std::memset(info, 0, sizeof(*info));
+    info->dtype = 0;
+    if (mesh == -1) {  // the whole batch
+        for (auto &m : b->meshes) info->nselements += m.nsel, info->nspoints += m.nsp;
+        info->maxdepth = b->maxdepth;
+        info->open_edges = -1;
+        info->device_bytes = b->device_bytes;
+        return SSDF_OK;
+    }
+    const DevMesh &m = b->meshes[mesh];
+    info->nselements = m.nsel;
+    info->nspoints = m.nsp;
+    info->maxdepth = m.hmeta.maxdepth;
+    info->open_edges = m.has_normals ? m.hmeta.open_edges : -1;
+    info->flipped_edges = m.hmeta.flipped_edges;
+    for (int d = 0; d < 6; d++) info->bbox[d] = ordered_to_float(m.hmeta.bbox_ord[d]);
+    info->device_bytes = 0;
+    return SSDF_OK;
+}
+
+}  // extern "C"

@@ -101,14 +101,12 @@ __device__ __forceinline__ T sign_dot(const T qx, const T qy, const T qz, const 
 // few outside it, which the FP64 evaluation rejects.  A list that stays full after the drop (more
 // than kCand triangles inside the band) sends the query to the exact FP64 traversal.
 // ------------------------------------------------------------------------------------------------
+// One wave's 64 queries (wave_id of the query source); `cand` is the block's LDS candidate table.
 template <int STACK, int QSRC, int OUT, bool FIX>
-__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
-                                                      QueryArgs a) {
+__device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+                                            const QueryArgs &a, const unsigned wave_id, int (*cand)[256]) {
     static_assert(STACK == 64 || STACK == 128, "the traversal stack lives in one or two VGPRs");
-    __shared__ int cand[FIX ? kCand : 1][256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
-    const unsigned wave_id = blk * kWaves + (unsigned)w;
+    const int lane = threadIdx.x & 63;
     long long qid = 0;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     const bool valid = lane_query<QSRC, float>(a, wave_id, lane, qid, qx, qy, qz);
@@ -246,10 +244,14 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
             // only the FP32 band: the FP64 minimum cannot lie outside it
             const unsigned pos = atomicAdd(a.fix_count, 1u);
             if (pos < a.fix_cap) {
-                a.fix_list[2 * pos] = (unsigned)qid;
-                a.fix_list[2 * pos + 1] = __float_as_uint(band_sq(bestd, tau1, babs));
+                a.fix_list[4 * pos] = (unsigned)qid;
+                a.fix_list[4 * pos + 1] = __float_as_uint(band_sq(bestd, tau1, babs));
+                a.fix_list[4 * pos + 2] = a.fix_tag;
+                a.fix_list[4 * pos + 3] = 0u;
+                return;
             }
-            return;
+            // queue full: keep the FP32 result and report it (error bit 4)
+            atomicOr(a.err_flag, 4);
         }
     }
 
@@ -307,6 +309,14 @@ __global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ n
     if (a.out_tri) a.out_tri[qid] = tr.orig;
 }
 
+template <int STACK, int QSRC, int OUT, bool FIX>
+__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+                                                      QueryArgs a) {
+    __shared__ int cand[FIX ? kCand : 1][256];
+    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
+    packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, blk * kWaves + (threadIdx.x >> 6), cand);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Exact FP64 traversal, one lane per query.  A box is skipped only when its FP64 distance exceeds
 // the best distance by delta64 (far above FP64 rounding of any distance), so the result is the
@@ -329,138 +339,275 @@ __device__ __forceinline__ void tri_verts64(const QueryArgs &a, const TriF &tr, 
 
 constexpr int kExactStack = 128;
 
-// QSRC: points / grid, or -1 for the fixup queue (query ids in fix_list, points or grid by GSRC).
+// One query, exact FP64: qid of the query source; band2 bounds the search (the FP32 band of a
+// queued fixup query, +inf otherwise).
+template <typename G, int QSRC, int OUT>
+__device__ void exact_one(const QueryArgs &a, const long long qid, const double band2) {
+    double qx, qy, qz;
+    if (QSRC == QSRC_POINTS) {
+        qx = (double)load_as<G>(a.qx, qid), qy = (double)load_as<G>(a.qy, qid), qz = (double)load_as<G>(a.qz, qid);
+    } else {
+        const long long i = qid % a.nx, j = (qid / a.nx) % a.ny, k = qid / ((long long)a.nx * a.ny);
+        const G hx = (G)a.gh[0], hy = (G)a.gh[1], hz = (G)a.gh[2];
+        qx = (double)((G)a.gmin[0] + (G)i * hx);
+        qy = (double)((G)a.gmin[1] + (G)j * hy);
+        qz = (double)((G)a.gmin[2] + (G)k * hz);
+    }
+    double out_init = __builtin_inf();
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
+    const double init2 = out_init * out_init;
+    double best = __builtin_inf(), bound = __builtin_inf();
+    int bt = INT_MAX, bk = -1;
+    const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
+    {
+        const double b0 = sqrt(fmin(init2, band2)) + delta;
+        bound = b0 * b0;
+    }
+    int st_node[kExactStack];
+    double st_d[kExactStack];
+    int sp = 0;
+    int node = a.meta->root;
+    long long iters = 0;
+    for (;;) {
+        if (++iters > a.max_iters) {
+            atomicOr(a.err_flag, 2);
+            break;
+        }
+        if (node >= 0) {
+            const Node nd = a.nodes[node];
+            const double dl = pt_box_d2<double>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
+                                                nd.lmax[1], nd.lmax[2]);
+            const double dr = pt_box_d2<double>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
+                                                nd.rmax[1], nd.rmax[2]);
+            const bool nl = dl <= bound, nr = dr <= bound;
+            if (nl && nr) {
+                const bool lfirst = dl <= dr;
+                if (sp < kExactStack) {
+                    st_node[sp] = lfirst ? nd.right : nd.left;
+                    st_d[sp] = lfirst ? dr : dl;
+                    sp++;
+                } else {
+                    atomicOr(a.err_flag, 1);
+                }
+                node = lfirst ? nd.left : nd.right;
+                continue;
+            }
+            if (nl) {
+                node = nd.left;
+                continue;
+            }
+            if (nr) {
+                node = nd.right;
+                continue;
+            }
+        } else {
+            const TriF tr = a.tris[~node].f;
+            double A[3], B[3], C[3];
+            tri_verts64<G>(a, tr, A, B, C);
+            const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
+            if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
+                best = d, bt = tr.orig, bk = ~node;
+                const double r = sqrt(best) + delta;
+                bound = r * r;
+            }
+        }
+        node = kNone;
+        while (sp > 0) {
+            --sp;
+            if (st_d[sp] <= bound) {
+                node = st_node[sp];
+                break;
+            }
+        }
+        if (node == kNone) break;
+    }
+    // outputs in caller precision G
+    const double dist = sqrt(best);
+    if (OUT == OUT_EDF) {
+        if (bk >= 0) {
+            store_as<G>(a.out_d, qid, (G)fmin(out_init, dist));
+            if (a.out_tri) a.out_tri[qid] = bt;
+        }
+        return;
+    }
+    if (bk < 0) {
+        if (OUT == OUT_SDF) {
+            store_as<G>(a.out_d, qid, (G)__builtin_inf());
+            if (a.out_tri) a.out_tri[qid] = -1;
+        }
+        return;
+    }
+    const TriF tr = a.tris[bk].f;
+    double A[3], B[3], C[3];
+    tri_verts64<G>(a, tr, A, B, C);
+    if (OUT == OUT_CLOSEST) {
+        const G gd = (G)dist;
+        if ((G)out_init > gd) {
+            double cx, cy, cz;
+            pt_tri_cp<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2], cx, cy, cz);
+            store_as<G>(a.out_d, qid, gd);
+            store_as<G>(a.out_cx, qid, (G)cx);
+            store_as<G>(a.out_cy, qid, (G)cy);
+            store_as<G>(a.out_cz, qid, (G)cz);
+            if (a.out_tri) a.out_tri[qid] = bt;
+        }
+        return;
+    }
+    double nl64;
+    const double dot = sign_dot<double>(qx, qy, qz, A, B, C, bt, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
+                                        a.vertex_n, nl64);
+    store_as<G>(a.out_d, qid, (G)(dot < 0 ? -dist : dist));
+    if (a.out_tri) a.out_tri[qid] = bt;
+}
+
+// QUEUE: the items are the fixup queue's entries (query id, FP32 band); else every query.
 template <typename G, int QSRC, int OUT, bool QUEUE>
 __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
-    long long qid;
-    double qx, qy, qz;
     const unsigned n_items = QUEUE ? min(*a.fix_count, a.fix_cap) : 0u;
     const long long total = QUEUE ? (long long)n_items
                                   : (QSRC == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz);
     for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
          item += (long long)gridDim.x * blockDim.x) {
-        double band2 = __builtin_inf();
-        if (QUEUE) {
-            qid = (long long)a.fix_list[2 * item];
-            band2 = (double)__uint_as_float(a.fix_list[2 * item + 1]);
-        } else {
-            qid = item;
-        }
-        if (QSRC == QSRC_POINTS) {
-            qx = (double)load_as<G>(a.qx, qid), qy = (double)load_as<G>(a.qy, qid), qz = (double)load_as<G>(a.qz, qid);
-        } else {
-            const long long i = qid % a.nx, j = (qid / a.nx) % a.ny, k = qid / ((long long)a.nx * a.ny);
-            const G hx = (G)a.gh[0], hy = (G)a.gh[1], hz = (G)a.gh[2];
-            qx = (double)((G)a.gmin[0] + (G)i * hx);
-            qy = (double)((G)a.gmin[1] + (G)j * hy);
-            qz = (double)((G)a.gmin[2] + (G)k * hz);
-        }
-        double out_init = __builtin_inf();
-        if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
-        const double init2 = out_init * out_init;
-        double best = __builtin_inf(), bound = __builtin_inf();
-        int bt = INT_MAX, bk = -1;
-        const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
-        {
-            const double b0 = sqrt(fmin(init2, band2)) + delta;
-            bound = b0 * b0;
-        }
-        int st_node[kExactStack];
-        double st_d[kExactStack];
-        int sp = 0;
-        int node = a.meta->root;
-        long long iters = 0;
-        for (;;) {
-            if (++iters > a.max_iters) {
-                atomicOr(a.err_flag, 2);
-                break;
-            }
-            if (node >= 0) {
-                const Node nd = a.nodes[node];
-                const double dl = pt_box_d2<double>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
-                                                    nd.lmax[1], nd.lmax[2]);
-                const double dr = pt_box_d2<double>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
-                                                    nd.rmax[1], nd.rmax[2]);
-                const bool nl = dl <= bound, nr = dr <= bound;
-                if (nl && nr) {
-                    const bool lfirst = dl <= dr;
-                    if (sp < kExactStack) {
-                        st_node[sp] = lfirst ? nd.right : nd.left;
-                        st_d[sp] = lfirst ? dr : dl;
-                        sp++;
-                    } else {
-                        atomicOr(a.err_flag, 1);
-                    }
-                    node = lfirst ? nd.left : nd.right;
-                    continue;
-                }
-                if (nl) {
-                    node = nd.left;
-                    continue;
-                }
-                if (nr) {
-                    node = nd.right;
-                    continue;
-                }
-            } else {
-                const TriF tr = a.tris[~node].f;
-                double A[3], B[3], C[3];
-                tri_verts64<G>(a, tr, A, B, C);
-                const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
-                if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
-                    best = d, bt = tr.orig, bk = ~node;
-                    const double r = sqrt(best) + delta;
-                    bound = r * r;
-                }
-            }
-            node = kNone;
-            while (sp > 0) {
-                --sp;
-                if (st_d[sp] <= bound) {
-                    node = st_node[sp];
-                    break;
-                }
-            }
-            if (node == kNone) break;
-        }
-        // outputs in caller precision G
-        const double dist = sqrt(best);
-        if (OUT == OUT_EDF) {
-            if (bk >= 0) {
-                store_as<G>(a.out_d, qid, (G)fmin(out_init, dist));
-                if (a.out_tri) a.out_tri[qid] = bt;
-            }
-            continue;
-        }
-        if (bk < 0) {
-            if (OUT == OUT_SDF) {
-                store_as<G>(a.out_d, qid, (G)__builtin_inf());
-                if (a.out_tri) a.out_tri[qid] = -1;
-            }
-            continue;
-        }
-        const TriF tr = a.tris[bk].f;
-        double A[3], B[3], C[3];
-        tri_verts64<G>(a, tr, A, B, C);
-        if (OUT == OUT_CLOSEST) {
-            const G gd = (G)dist;
-            if ((G)out_init > gd) {
-                double cx, cy, cz;
-                pt_tri_cp<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2], cx, cy, cz);
-                store_as<G>(a.out_d, qid, gd);
-                store_as<G>(a.out_cx, qid, (G)cx);
-                store_as<G>(a.out_cy, qid, (G)cy);
-                store_as<G>(a.out_cz, qid, (G)cz);
-                if (a.out_tri) a.out_tri[qid] = bt;
-            }
-            continue;
-        }
-        double nl64;
-        const double dot = sign_dot<double>(qx, qy, qz, A, B, C, bt, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
-                                            a.vertex_n, nl64);
-        store_as<G>(a.out_d, qid, (G)(dot < 0 ? -dist : dist));
-        if (a.out_tri) a.out_tri[qid] = bt;
+        if (QUEUE)
+            exact_one<G, QSRC, OUT>(a, (long long)a.fix_list[4 * item],
+                                    (double)__uint_as_float(a.fix_list[4 * item + 1]));
+        else
+            exact_one<G, QSRC, OUT>(a, item, __builtin_inf());
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batches
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void batch_args(const BatchArgs &b, const BatchMesh &bm, QueryArgs &a) {
+    a = b.base;
+    a.nodes = b.nodes + bm.node_off;
+    a.tris = b.tris + bm.tri_off;
+    a.meta = b.metas + bm.mesh;
+    a.s0 = b.s0 + bm.tri_off, a.s1 = b.s1 + bm.tri_off, a.s2 = b.s2 + bm.tri_off;
+    a.sx = b.sx + bm.vert_off, a.sy = b.sy + bm.vert_off, a.sz = b.sz + bm.vert_off;
+    a.face_n = b.face_n ? b.face_n + 3 * bm.tri_off : nullptr;
+    a.edge_n = b.edge_n ? b.edge_n + 9 * bm.tri_off : nullptr;
+    a.vertex_n = b.vertex_n ? b.vertex_n + 3 * bm.vert_off : nullptr;
+    a.qx = b.qx ? b.qx + bm.out_off : nullptr;
+    a.qy = b.qy ? b.qy + bm.out_off : nullptr;
+    a.qz = b.qz ? b.qz + bm.out_off : nullptr;
+    a.npoints = bm.npoints;
+    a.nx = bm.nx, a.ny = bm.ny, a.nz = bm.nz, a.nbx = bm.nbx, a.nby = bm.nby, a.nbz = bm.nbz;
+#pragma unroll
+    for (int d = 0; d < 3; d++) a.gmin[d] = bm.gmin[d], a.gh[d] = bm.gh[d];
+    a.out_d = b.out_d + bm.out_off;
+    a.out_tri = b.out_tri ? b.out_tri + bm.out_off : nullptr;
+    a.out_cx = b.out_cx ? b.out_cx + bm.out_off : nullptr;
+    a.out_cy = b.out_cy ? b.out_cy + bm.out_off : nullptr;
+    a.out_cz = b.out_cz ? b.out_cz + bm.out_off : nullptr;
+    a.band_abs = bm.band_abs, a.sign_eps = bm.sign_eps, a.delta64 = bm.delta64;
+    a.fix_tag = (unsigned)bm.mesh;
+}
+
+// Persistent: every wave takes `chunk` units at a time from a work queue until all are done.  The
+// units are split into 8 contiguous ranges with a counter each; block b starts on range b % 8 (the
+// XCD the hardware dispatches it to, so each XCD's L2 holds mostly its range's meshes) and moves
+// on to the next ranges when its own is exhausted.
+template <int STACK, int QSRC, int OUT, bool FIX>
+__global__ __launch_bounds__(256) void k_batch_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+                                                      const BatchMesh *__restrict__ meshes, BatchArgs b) {
+    __shared__ int cand[FIX ? kCand : 1][256];
+    const int lane = threadIdx.x & 63;
+    const unsigned U = b.total_units;
+    const unsigned x0 = blockIdx.x & 7u;
+    for (unsigned k = 0; k < 8; k++) {
+        const unsigned x = (x0 + k) & 7u;
+        const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
+        const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
+        for (;;) {
+            unsigned grab = 0;
+            if (lane == 0) grab = atomicAdd(&b.queue[64 * x], b.chunk);
+            grab = (unsigned)uniform_i((int)__shfl((int)grab, 0));
+            const unsigned u0 = lo + grab;
+            if (grab >= hi - lo) break;
+            const unsigned u1 = min(u0 + b.chunk, hi);
+            for (unsigned u = u0; u < u1; u++) {
+                int m0 = 0, m1 = b.nmeshes;
+                while (m1 - m0 > 1) {
+                    const int mid = (m0 + m1) >> 1;
+                    if (meshes[mid].unit_start <= u)
+                        m0 = mid;
+                    else
+                        m1 = mid;
+                }
+                const BatchMesh &bm = meshes[m0];
+                QueryArgs a;
+                batch_args(b, bm, a);
+                packet_wave<STACK, QSRC, OUT, FIX>(nodes + bm.node_off, tris + bm.tri_off, a, u - bm.unit_start,
+                                                   cand);
+            }
+        }
+    }
+}
+
+template <int QSRC, int OUT>
+__global__ __launch_bounds__(64) void k_batch_exact(const BatchMesh *__restrict__ meshes, BatchArgs b) {
+    const unsigned n_items = min(*b.base.fix_count, b.base.fix_cap);
+    for (unsigned item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+        const unsigned *e = b.base.fix_list + 4 * (size_t)item;
+        const int mi = (int)e[2];
+        // descriptors are ordered by unit, not by mesh: find the mesh's entry
+        int j = 0;
+        while (j < b.nmeshes - 1 && meshes[j].mesh != mi) j++;
+        QueryArgs a;
+        batch_args(b, meshes[j], a);
+        exact_one<float, QSRC, OUT>(a, (long long)e[0], (double)__uint_as_float(e[1]));
+    }
+}
+
+__global__ void k_batch_reset(unsigned *queue, unsigned *status) {
+    if (threadIdx.x < 8) queue[64 * threadIdx.x] = 0u;
+    if (threadIdx.x < 2) status[threadIdx.x] = 0u;
+}
+
+template <int STACK, int QSRC, int OUT>
+static void launch_batch_packet(const BatchArgs &b, const BatchMesh *meshes, bool fix, unsigned nblocks,
+                                hipStream_t s) {
+    if (fix)
+        hipLaunchKernelGGL((k_batch_packet<STACK, QSRC, OUT, true>), dim3(nblocks), dim3(256), 0, s, b.nodes, b.tris,
+                           meshes, b);
+    else
+        hipLaunchKernelGGL((k_batch_packet<STACK, QSRC, OUT, false>), dim3(nblocks), dim3(256), 0, s, b.nodes,
+                           b.tris, meshes, b);
+    if (fix) hipLaunchKernelGGL((k_batch_exact<QSRC, OUT>), dim3(256), dim3(64), 0, s, meshes, b);
+}
+
+template <int STACK, int QSRC>
+static int launch_batch_out(const BatchArgs &b, const BatchMesh *meshes, int out, bool fix, unsigned nblocks,
+                            hipStream_t s) {
+    if (out == OUT_EDF)
+        launch_batch_packet<STACK, QSRC, OUT_EDF>(b, meshes, fix, nblocks, s);
+    else if (out == OUT_SDF)
+        launch_batch_packet<STACK, QSRC, OUT_SDF>(b, meshes, fix, nblocks, s);
+    else if (QSRC == QSRC_POINTS)
+        launch_batch_packet<STACK, QSRC_POINTS, OUT_CLOSEST>(b, meshes, fix, nblocks, s);
+    else
+        return SSDF_ERR_UNSUPPORTED;
+    return SSDF_OK;
+}
+
+int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out, bool fix, int stack,
+                 unsigned nblocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, b.queue, b.base.fix_count);
+    int rc;
+    if (stack == 64)
+        rc = qsrc == QSRC_POINTS ? launch_batch_out<64, QSRC_POINTS>(b, meshes, out, fix, nblocks, stream)
+                                 : launch_batch_out<64, QSRC_GRID>(b, meshes, out, fix, nblocks, stream);
+    else
+        rc = qsrc == QSRC_POINTS ? launch_batch_out<128, QSRC_POINTS>(b, meshes, out, fix, nblocks, stream)
+                                 : launch_batch_out<128, QSRC_GRID>(b, meshes, out, fix, nblocks, stream);
+    if (rc) {
+        set_error("batch: closest-point output needs point queries");
+        return rc;
+    }
+    SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -4,6 +4,9 @@
 #include "common.hpp"
 #include "mesh.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace ssdf_amd {
 
 enum QuerySource : int { QSRC_POINTS = 0, QSRC_GRID = 1 };
@@ -46,9 +49,10 @@ struct QueryArgs {
     double delta64; // FP64 pruning margin of the exact traversal
     // fixup queue: query ids whose candidate list overflowed
     unsigned *fix_count;
-    unsigned *fix_list;
-    unsigned fix_cap;
-    int *err_flag;  // bit 1: a traversal stack would overflow; bit 2: iteration cap hit
+    unsigned *fix_list;  // 4 words per entry: query id, FP32 band (bits), tag (mesh of a batch), 0
+    unsigned fix_cap;    // entries
+    unsigned fix_tag;
+    int *err_flag;  // bit 1: a traversal stack would overflow; bit 2: iteration cap hit; bit 4: fixup queue full
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
     unsigned nblocks;  // launched blocks (XCD remap)
     unsigned flags;    // QF_* of the call
@@ -61,6 +65,68 @@ struct QueryArgs {
 
 int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,
                  int q_is_double, hipStream_t stream);
+
+// Precision constants of the queries (single meshes and batches alike).
+constexpr float kBandRel = 0x1p-18f;    // FP32 error bound of a distance, relative to the coordinate scale
+constexpr float kTau = 1e-5f;           // relative near-tie band on distances
+constexpr double kDelta64Rel = 1e-13;   // FP64 pruning margin relative to the coordinate scale
+constexpr double kSignEpsRel = 1e-6;    // FP64 sign within this fraction of the bbox diagonal
+
+// Coordinate scale max(1, |vertex bounds|) and bounding-box diagonal of a built mesh.
+inline float meta_scale(const MeshMeta &hm) {
+    float L = 1.0f;
+    for (int d = 0; d < 6; d++) L = std::max(L, std::fabs(ordered_to_float(hm.bbox_ord[d])));
+    return L;
+}
+
+inline float meta_diag(const MeshMeta &hm) {
+    double s = 0;
+    for (int d = 0; d < 3; d++) {
+        const double e = (double)ordered_to_float(hm.bbox_ord[3 + d]) - ordered_to_float(hm.bbox_ord[d]);
+        s += e * e;
+    }
+    return (float)std::sqrt(s);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batches: many FP32 meshes in shared arenas, their queries answered by one persistent launch.
+// Work unit = one wave's query set (a 4x4x4 brick of a mesh's grid, or 64 of its points); units
+// are numbered mesh after mesh in the order of the descriptor array.
+// ------------------------------------------------------------------------------------------------
+struct BatchMesh {
+    long long node_off, tri_off, vert_off;  // arena offsets (elements) of this mesh's records
+    long long out_off;                      // offset of its first query in the query / output arrays
+    long long npoints;                      // points mode
+    int nx, ny, nz, nbx, nby, nbz;          // grid mode
+    double gmin[3], gh[3];
+    float band_abs, sign_eps;
+    double delta64;
+    unsigned unit_start;  // first work unit
+    int mesh;             // index of the mesh in the batch (meta record, fixup tag)
+};
+
+struct BatchArgs {
+    QueryArgs base;  // fields shared by every mesh: tolerances, flags, fixup queue, error flag
+    const Node *nodes;
+    const TriP *tris;
+    const MeshMeta *metas;
+    const int *s0, *s1, *s2;
+    const float *sx, *sy, *sz;
+    const double *face_n, *edge_n, *vertex_n;
+    const float *qx, *qy, *qz;  // points mode, concatenated per mesh at out_off
+    float *out_d;
+    int *out_tri;
+    float *out_cx, *out_cy, *out_cz;
+    int nmeshes;
+    unsigned total_units;
+    unsigned *queue;  // 8 counters (one per XCD range), 64 words apart
+    unsigned chunk;   // units per grab
+};
+
+// Launches the reset, the persistent packet kernel and (FIX) the fixup-queue kernel of a batch.
+// meshes: device descriptor array of b.nmeshes entries; stack: 64 or 128.
+int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out, bool fix, int stack,
+                 unsigned nblocks, hipStream_t stream);
 
 // Generalized winding number of the queries (gwn.hip): GW_WINDING writes w (float) to out_w,
 // GW_INSIDE writes w >= 1/2 (uint8) to out_in, GW_SIGN gives out_d (mesh precision) the sign

@@ -1,0 +1,167 @@
+"""GPU batches: many meshes in shared arenas, one persistent launch per call.  Every mesh's batched
+result must be bit-identical to the single-mesh path (same kernels, per-mesh offsets), which the
+parity tests pin to the oracle; one mesh is also checked against the oracle directly.  Captured
+graphs are replayed several times, with eager work between replays, against the eager result."""
+import numpy as np
+import pytest
+
+import ssdf_amd as S
+from ssdf_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    ms = [mg.uv_sphere(0.5, 24, 12), mg.torus(0.6, 0.2, 80, 30, sigma=0.004, seed=3),
+          mg.displaced_icosphere(3000, seed=1), mg.displaced_icosphere(12000, seed=2),
+          mg.Mesh(np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32),
+                  np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32)),
+          mg.displaced_icosphere(1000, seed=3)]
+    return ms
+
+
+def _grids(ms):
+    dims, boxes = [], []
+    for i, m in enumerate(ms):
+        if m.ntris:
+            lo, hi = m.bbox()
+        else:
+            lo, hi = np.array([-1.0, -1, -1]), np.array([1.0, 1, 1])
+        pad = 0.1 * (hi - lo)
+        dims.append((9 + 7 * i, 13 + 3 * i, 11 + 5 * i))
+        boxes.append(np.concatenate([lo - pad, hi + pad]))
+    return np.array(dims), np.array(boxes)
+
+
+def _single_grid(m, n, box, flags):
+    N = n[0] * n[1] * n[2]
+    if m.ntris == 0:
+        return None, None
+    rm = S.ResidentMesh.from_mesh(m)
+    o = S.DeviceArray(N, np.float32)
+    t = S.DeviceArray(N, np.int32)
+    rm.query_grid(S.MODE_SDF, tuple(int(v) for v in n), box[:3], box[3:], o, t, flags=flags)
+    return o.download(), t.download()
+
+
+@pytest.mark.parametrize("flags", [S.FLAG_FIXUP, 0])
+def test_batch_grids_equal_single_mesh(meshes, flags):
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(meshes)
+    out = S.DeviceArray(total, np.float32)
+    tri = S.DeviceArray(total, np.int32)
+    out.upload(np.full(total, 123.0, np.float32))
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags)
+    q, err = b.last_stats()
+    assert err == 0
+    o, t = out.download(), tri.download()
+    for i, m in enumerate(meshes):
+        n = dims[i]
+        N = int(n.prod())
+        so, st = _single_grid(m, n, boxes[i], flags)
+        if so is None:
+            assert np.all(o[off[i]:off[i] + N] == 123.0)  # empty mesh: untouched
+            continue
+        assert np.array_equal(o[off[i]:off[i] + N], so), i
+        assert np.array_equal(t[off[i]:off[i] + N], st), i
+
+
+def test_batch_mesh_against_oracle(meshes, oracle):
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(meshes)
+    out = S.DeviceArray(total, np.float32)
+    tri = S.DeviceArray(total, np.int32)
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+    i = 3
+    m, n = meshes[i], dims[i]
+    x, y, z = mg.grid_points(*n, boxes[i][:3].astype(np.float32), boxes[i][3:].astype(np.float32))
+    s, rtri, _, _, _ = oracle.sdf(x, y, z, m)
+    N = int(n.prod())
+    assert np.array_equal(tri.download()[off[i]:off[i] + N], rtri)
+    lo, hi = m.bbox()
+    eps = 1e-6 * float(np.linalg.norm(hi - lo))
+    o = out.download()[off[i]:off[i] + N]
+    assert np.abs(np.abs(o) - np.abs(s)).max() <= eps
+
+
+def test_batch_points_closest_and_edf(meshes):
+    rng = np.random.default_rng(4)
+    counts = [300, 1000, 65, 0, 500, 129]
+    po = np.zeros(len(meshes) + 1, np.int64)
+    po[1:] = np.cumsum(counts)
+    P = rng.uniform(-1.2, 1.2, (int(po[-1]), 3)).astype(np.float32)
+    x, y, z = (S.DeviceArray.from_host(np.ascontiguousarray(P[:, d])) for d in range(3))
+    b = S.Batch(meshes)
+    Np = int(po[-1])
+    init = np.full(Np, 0.35, np.float32)
+    od = S.DeviceArray.from_host(init)
+    ot = S.DeviceArray.from_host(np.full(Np, -5, np.int32))
+    cx, cy, cz = (S.DeviceArray.from_host(np.zeros(Np, np.float32)) for _ in range(3))
+    b.query_points(S.MODE_CLOSEST, po, x, y, z, od, ot, cx, cy, cz, flags=S.FLAG_FIXUP)
+    e = S.DeviceArray.from_host(init)
+    b.query_points(S.MODE_EDF, po, x, y, z, e, None, flags=0)
+    got_d, got_t, got_e = od.download(), ot.download(), e.download()
+    for i, m in enumerate(meshes):
+        sl = slice(int(po[i]), int(po[i + 1]))
+        if m.ntris == 0 or counts[i] == 0:
+            assert np.all(got_d[sl] == 0.35) and np.all(got_t[sl] == -5)
+            continue
+        rm = S.ResidentMesh.from_mesh(m)
+        px, py, pz = (S.DeviceArray.from_host(np.ascontiguousarray(P[sl, d])) for d in range(3))
+        d1 = S.DeviceArray.from_host(init[sl])
+        t1 = S.DeviceArray.from_host(np.full(counts[i], -5, np.int32))
+        c1 = [S.DeviceArray.from_host(np.zeros(counts[i], np.float32)) for _ in range(3)]
+        rm.query_points(S.MODE_CLOSEST, px, py, pz, d1, t1, *c1, flags=S.FLAG_FIXUP)
+        assert np.array_equal(got_d[sl], d1.download()) and np.array_equal(got_t[sl], t1.download())
+        e1 = S.DeviceArray.from_host(init[sl])
+        rm.query_points(S.MODE_EDF, px, py, pz, e1, None, flags=0)
+        assert np.array_equal(got_e[sl], e1.download())
+
+
+def test_batch_graph_replays_match_eager(meshes):
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(meshes)
+    st = S.Stream()
+    eo = S.DeviceArray.from_host(np.zeros(total, np.float32))  # the empty mesh's slice stays untouched
+    et = S.DeviceArray.from_host(np.zeros(total, np.int32))
+    b.query_grids(S.MODE_SDF, dims, boxes, off, eo, et, flags=S.FLAG_FIXUP, stream=st)
+    st.synchronize()
+    want_o, want_t = eo.download(), et.download()
+    go = S.DeviceArray(total, np.float32)
+    gt = S.DeviceArray(total, np.int32)
+    b.capture(True, S.MODE_SDF, dims, boxes, off, go, gt, flags=S.FLAG_FIXUP, stream=st)
+    for rep in range(3):
+        go.upload(np.zeros(total, np.float32))
+        gt.upload(np.zeros(total, np.int32))
+        b.replay(st)
+        st.synchronize()
+        assert np.array_equal(go.download(), want_o), rep
+        assert np.array_equal(gt.download(), want_t), rep
+        # eager work between replays
+        b.query_grids(S.MODE_SDF, dims, boxes, off, eo, et, flags=0, stream=st)
+        st.synchronize()
+
+
+def test_batch_autotune_and_schedule(meshes):
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(meshes)
+    out = S.DeviceArray.from_host(np.zeros(total, np.float32))  # the empty mesh's slice stays untouched
+    tri = S.DeviceArray.from_host(np.zeros(total, np.int32))
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+    want = out.download(), tri.download()
+    bpc, ch, ms = b.autotune(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+    assert bpc > 0 and ch > 0 and ms > 0
+    for sched in [(1, 1), (8, 16), (0, 0)]:
+        b.set_schedule(*sched)
+        out.upload(np.zeros(total, np.float32))
+        b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+        assert np.array_equal(out.download(), want[0]) and np.array_equal(tri.download(), want[1])
+    info = b.info()
+    assert info["nselements"] == sum(m.ntris for m in meshes)
+    with pytest.raises(S.SsdfError):
+        b.set_schedule(99, 1)
