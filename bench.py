@@ -134,6 +134,65 @@ def run_gpu(args, m, n, lo, hi):
     return res
 
 
+def run_batch(args):
+    """64 displaced icospheres (4.1 M triangles), one grid each (84.8 M nodes), signed distance +
+    triangle with FP64 fixup: the per-mesh reference-shaped call vs one batch."""
+    import ssdf_amd as S
+    from ssdf_amd import workloads as W
+    meshes, dims, boxes = W.batch64()
+    nq = int(dims.prod(1).sum())
+    flags = S.FLAG_FIXUP
+    res = {"meshes": len(meshes), "triangles": int(sum(m.ntris for m in meshes)), "queries": nq}
+    # per-mesh plain path: upload, build, query, download, free per call (serial/bvh/bvh.cu:80-158)
+    outs = [np.empty(int(d.prod()), np.float32) for d in dims]
+    t0 = time.perf_counter()
+    for m, d, bx, o in zip(meshes, dims, boxes, outs):
+        S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, int(d[0]), int(d[1]), int(d[2]), *bx.astype(np.float32),
+                         o, None, flags=flags)
+    res["per_mesh_calls_s"] = time.perf_counter() - t0
+    res["per_mesh_calls_mqps"] = nq / res["per_mesh_calls_s"] / 1e6
+    log(f"[bench] batch64 per-mesh calls: {res['per_mesh_calls_s']:.3f} s")
+    # one batch, host to host: upload + build all, one query launch, download
+    off, total = S.Batch.grid_layout(dims)
+    st = S.Stream()
+    out = S.DeviceArray(total, np.float32)
+    tri = S.DeviceArray(total, np.int32)
+    host = np.empty(total, np.float32)
+    t0 = time.perf_counter()
+    b = S.Batch(meshes, stream=st)
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+    st.synchronize()
+    host[:] = out.download()
+    res["batch_host_to_host_s"] = time.perf_counter() - t0
+    res["batch_host_to_host_mqps"] = nq / res["batch_host_to_host_s"] / 1e6
+    # the per-mesh calls and the batch agree bit for bit
+    res["matches_per_mesh_calls"] = bool(all(np.array_equal(host[off[i]:off[i] + outs[i].size], outs[i])
+                                             for i in range(len(meshes))))
+    bpc, ch, tune_ms = b.autotune(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+    res["schedule"] = {"blocks_per_cu": bpc, "units_per_grab": ch, "query_ms": round(tune_ms, 3)}
+    ev = [S.Event(), S.Event()]
+    for rebuild, key in ((True, "resident_step"), (False, "resident_query")):
+        b.capture(rebuild, S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+        for _ in range(max(1, args.warmup)):
+            b.replay(st)
+        st.synchronize()
+        ev[0].record(st)
+        for _ in range(args.steps):
+            b.replay(st)
+        ev[1].record(st)
+        st.synchronize()
+        ms = ev[0].elapsed_ms(ev[1]) / args.steps
+        res[key + "_ms"] = ms
+        res[key + "_mqps"] = nq / (ms * 1e-3) / 1e6
+        log(f"[bench] batch64 {key}: {ms:.3f} ms")
+    q, e = b.last_stats()
+    res["fixup_queued"], res["kernel_error"] = q, e
+    res["speedup_resident_step_vs_per_mesh_calls"] = res["resident_step_mqps"] / res["per_mesh_calls_mqps"]
+    res["speedup_host_to_host_vs_per_mesh_calls"] = res["batch_host_to_host_mqps"] / res["per_mesh_calls_mqps"]
+    b.close()
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
+
+
 def cpu_baseline(args, m, n, lo, hi):
     """The oracle's edf_celllist<float> restatement (serial/ssdf.hpp:642-963) on a seeded uniform
     subsample of the grid, all host threads."""
@@ -194,6 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fixup", "plain"], default=None,
                     help="time one mode only and print nothing else (for profiling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-mesh batch leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -232,6 +292,8 @@ def main():
     }
     if args.grid == 256:
         line["roofline"] = roofline(g["fixup"]["query_ms"], N, m.nverts, m.ntris)
+    if not args.no_batch:
+        line["batch64"] = run_batch(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
     print(json.dumps(line), flush=True)

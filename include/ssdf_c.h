@@ -244,6 +244,46 @@ int ssdf_batch_last_stats(ssdf_batch_t batch, unsigned *fixup_queued, int *kerne
 int ssdf_batch_info(ssdf_batch_t batch, int mesh, ssdf_mesh_info_t *info);
 
 /* ---------------------------------------------------------------------------------------------
+ * Serving (no reference counterpart).  A worker thread keeps registered meshes resident (built on
+ * first use, least recently used idle meshes evicted when the HBM budget needs room), admits
+ * requests in arrival order when their device buffers (20 B per query) fit the budget, and runs
+ * them as soon as they are admitted on a pool of streams (upload, query, download).  Requests
+ * take HOST arrays that must stay valid until the request completes; pinned memory
+ * (ssdf_host_alloc) lets the copies run asynchronously.
+ * --------------------------------------------------------------------------------------------- */
+typedef struct ssdf_server *ssdf_server_t;
+
+typedef struct {
+    unsigned long long served, queries, errors, mesh_loads, evictions;
+    size_t pending, inflight, bytes_in_use, peak_bytes;
+    double p50_s, p99_s, max_s; /* latency: arrival to results in host memory */
+    double throughput_qps;      /* queries / (last completion - first arrival) */
+} ssdf_server_stats_t;
+
+/* max_inflight: requests on the device at once (0 = 16). */
+int ssdf_server_create(size_t hbm_budget_bytes, int max_inflight, ssdf_server_t *out);
+/* Registers an FP32 mesh (host arrays kept by the caller for the server's lifetime). */
+int ssdf_server_add_mesh(ssdf_server_t server, int mesh_id, ptrdiff_t nselements, const int *s0, const int *s1,
+                         const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy, const float *sz);
+/* Seconds on the server's clock (its creation is 0). */
+int ssdf_server_clock(ssdf_server_t server, double *now);
+/* mode EDF (out_d and out_tri read as the current best, kept where not improved) or SDF (both
+ * overwritten); arrival: timestamp on the server clock for the
+ * latency (<= 0: now).  Returns a ticket for ssdf_server_wait / ssdf_server_poll. */
+int ssdf_server_submit(ssdf_server_t server, int mesh_id, int mode, ptrdiff_t npoints, const float *x,
+                       const float *y, const float *z, float *out_d, int *out_tri, unsigned flags, double arrival,
+                       uint64_t *ticket);
+/* Blocks until the request completed; releases the ticket. */
+int ssdf_server_wait(ssdf_server_t server, uint64_t ticket, double *latency_s);
+int ssdf_server_poll(ssdf_server_t server, uint64_t ticket, int *done);
+int ssdf_server_stats(ssdf_server_t server, ssdf_server_stats_t *stats);
+/* Drains the queue, then stops the worker and frees the resident meshes. */
+int ssdf_server_destroy(ssdf_server_t server);
+/* Pinned host memory for request buffers. */
+int ssdf_host_alloc(void **ptr, size_t bytes);
+int ssdf_host_free(void *ptr);
+
+/* ---------------------------------------------------------------------------------------------
  * Device plumbing for callers without a GPU runtime of their own (tests, benchmarks, FFI).
  * --------------------------------------------------------------------------------------------- */
 int ssdf_device_count(int *count);

@@ -23,7 +23,7 @@ __all__ = [
     "sample_sdf_bvh", "sdf", "ResidentMesh", "DeviceArray", "Stream", "Event", "FLAG_FIXUP", "FLAG_SIGN_FP64",
     "FLAG_SIGN_GWN", "FLAG_SIGN_AUTO", "points_inside_bvh", "sample_inside_bvh", "compute_aabb",
     "all_points_aabb_signed_distance", "init_volume_box",
-    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST", "Batch",
+    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST", "Batch", "Server", "PinnedArray",
 ]
 
 FLAG_FIXUP = 1
@@ -43,6 +43,13 @@ class SsdfError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str = ""):
         super().__init__(f"{where} failed with code {code}: {msg}")
         self.code = code
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("served", C.c_ulonglong), ("queries", C.c_ulonglong), ("errors", C.c_ulonglong),
+                ("mesh_loads", C.c_ulonglong), ("evictions", C.c_ulonglong), ("pending", C.c_size_t),
+                ("inflight", C.c_size_t), ("bytes_in_use", C.c_size_t), ("peak_bytes", C.c_size_t),
+                ("p50_s", C.c_double), ("p99_s", C.c_double), ("max_s", C.c_double), ("throughput_qps", C.c_double)]
 
 
 class _MeshInfo(C.Structure):
@@ -95,6 +102,17 @@ _SIGS = {
     "ssdf_batch_replay": [_P, _P],
     "ssdf_batch_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
     "ssdf_batch_info": [_P, C.c_int, C.POINTER(_MeshInfo)],
+    "ssdf_server_create": [C.c_size_t, C.c_int, C.POINTER(_P)],
+    "ssdf_server_add_mesh": [_P, C.c_int, _PD, _P, _P, _P, _PD, _P, _P, _P],
+    "ssdf_server_clock": [_P, C.POINTER(C.c_double)],
+    "ssdf_server_submit": [_P, C.c_int, C.c_int, _PD, _P, _P, _P, _P, _P, C.c_uint, C.c_double,
+                           C.POINTER(C.c_uint64)],
+    "ssdf_server_wait": [_P, C.c_uint64, C.POINTER(C.c_double)],
+    "ssdf_server_poll": [_P, C.c_uint64, C.POINTER(C.c_int)],
+    "ssdf_server_stats": [_P, C.c_void_p],
+    "ssdf_server_destroy": [_P],
+    "ssdf_host_alloc": [C.POINTER(_P), C.c_size_t],
+    "ssdf_host_free": [_P],
     "ssdf_device_count": [C.POINTER(C.c_int)],
     "ssdf_device_malloc": [C.POINTER(_P), C.c_size_t],
     "ssdf_device_free": [_P],
@@ -574,6 +592,89 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().ssdf_batch_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------------------------
+# serving
+# ------------------------------------------------------------------------------------------------
+class PinnedArray:
+    """Pinned host memory viewed as a numpy array (request buffers the server copies asynchronously)."""
+
+    def __init__(self, n: int, dtype):
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        _check(lib().ssdf_host_alloc(C.byref(p), max(1, int(n) * self.dtype.itemsize)), "ssdf_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * (max(1, int(n)) * self.dtype.itemsize)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(n))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().ssdf_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over resident meshes (HBM-budget admission, continuous batching, p50/p99)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 0):
+        h = C.c_void_p()
+        _check(lib().ssdf_server_create(int(hbm_budget_bytes), int(max_inflight), C.byref(h)), "ssdf_server_create")
+        self.h = h.value
+        self._keep = {}
+
+    def add_mesh(self, mesh_id: int, m):
+        arrs = [np.ascontiguousarray(a) for a in (m.s0, m.s1, m.s2, m.sx, m.sy, m.sz)]
+        self._keep[("mesh", mesh_id)] = arrs
+        rc = lib().ssdf_server_add_mesh(C.c_void_p(self.h), mesh_id, arrs[0].size, *[_ptr(a) for a in arrs[:3]],
+                                        arrs[3].size, *[_ptr(a) for a in arrs[3:]])
+        _check(rc, "ssdf_server_add_mesh")
+
+    def clock(self) -> float:
+        t = C.c_double()
+        _check(lib().ssdf_server_clock(C.c_void_p(self.h), C.byref(t)), "ssdf_server_clock")
+        return t.value
+
+    def submit(self, mesh_id, mode, x, y, z, out_d, out_tri=None, flags=0, arrival: float = 0.0) -> int:
+        """x, y, z, out_d, out_tri: host arrays (numpy or PinnedArray.array) alive until wait()."""
+        t = C.c_uint64()
+        rc = lib().ssdf_server_submit(C.c_void_p(self.h), mesh_id, mode, x.size, _ptr(x), _ptr(y), _ptr(z),
+                                      _ptr(out_d), _ptr(out_tri), flags, float(arrival), C.byref(t))
+        _check(rc, "ssdf_server_submit")
+        return int(t.value)
+
+    def wait(self, ticket: int) -> float:
+        lat = C.c_double()
+        _check(lib().ssdf_server_wait(C.c_void_p(self.h), ticket, C.byref(lat)), "ssdf_server_wait")
+        return lat.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int()
+        _check(lib().ssdf_server_poll(C.c_void_p(self.h), ticket, C.byref(d)), "ssdf_server_poll")
+        return bool(d.value)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().ssdf_server_stats(C.c_void_p(self.h), C.c_void_p(C.addressof(st))), "ssdf_server_stats")
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ssdf_server_destroy(C.c_void_p(self.h))
             self.h = None
 
     def __del__(self):

@@ -477,6 +477,19 @@ int host_inside_grid(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2
 
 }  // namespace
 
+namespace ssdf_amd {
+// Resident FP32 mesh with normals from host arrays, for the serving loop (server.hip).
+int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t nsel, const float *sx,
+                       const float *sy, const float *sz, ptrdiff_t nsp, hipStream_t stream, ssdf_mesh_t *out,
+                       size_t *bytes) {
+    ssdf_mesh *h = nullptr;
+    if (int rc = create_mesh(0, nsel, s0, s1, s2, nsp, sx, sy, sz, false, true, stream, &h)) return rc;
+    *out = h;
+    *bytes = h->m.device_bytes + h->m.ws_cap;
+    return SSDF_OK;
+}
+}  // namespace ssdf_amd
+
 extern "C" {
 
 int ssdf_points_inside_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,

@@ -207,9 +207,14 @@ def displaced_icosphere(ntris_target: int, seed: int, amplitude: float = 0.1, de
     rng = SplitMix64(seed)
     terms = [(a, b, c) for a in range(degree + 1) for b in range(degree + 1 - a) for c in range(degree + 1 - a - b)]
     coef = rng.normal(len(terms))
+    # power tables: x^a by repeated products
+    P = [[np.ones(v.shape[0])] for _ in range(3)]
+    for d in range(3):
+        for _ in range(degree):
+            P[d].append(P[d][-1] * v[:, d])
     f = np.zeros(v.shape[0])
     for (a, b, c), w in zip(terms, coef):
-        f += w * v[:, 0] ** a * v[:, 1] ** b * v[:, 2] ** c
+        f += w * (P[0][a] * P[1][b] * P[2][c])
     f /= np.max(np.abs(f))
     v *= (1.0 + amplitude * f)[:, None]
     faces = np.stack([m.s0, m.s1, m.s2], axis=1)
