@@ -193,6 +193,84 @@ def run_batch(args):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
 
 
+def run_serve(args):
+    """Open-loop serving: Poisson arrivals (200 requests/s) over a pool of 32 resident meshes
+    (5 k - 2 M triangles), 1e4 - 1e7 signed queries per request (half uniform in the padded bbox,
+    half within 0.02 of the surface), FP32 + fixup; latency from the scheduled arrival to the
+    results in host memory."""
+    import ssdf_amd as S
+    from ssdf_amd import workloads as W
+    t_gen = time.perf_counter()
+    meshes = W.serve_pool()
+    log(f"[bench] serve: mesh pool generated ({time.perf_counter() - t_gen:.1f} s)")
+    times, mesh_ids, nq = W.poisson_requests(200.0, args.serve_seconds, len(meshes))
+    nmax = int(nq.max())
+    base = 1 << 20
+    pools = []
+    for i, m in enumerate(meshes):
+        pts = W.query_cloud(m, base, seed=100 + i)
+        arrs = []
+        for v in pts:
+            pa = S.PinnedArray(nmax, np.float32)
+            reps = -(-nmax // base)
+            pa.array[:] = np.tile(v, reps)[:nmax]
+            arrs.append(pa)
+        pools.append(arrs)
+    ring = 24
+    outs = [(S.PinnedArray(nmax, np.float32), S.PinnedArray(nmax, np.int32)) for _ in range(ring)]
+    gen_s = time.perf_counter() - t_gen
+    log(f"[bench] serve: query pools and output ring pinned ({gen_s:.1f} s)")
+    budget = 32 << 30
+    srv = S.Server(budget, max_inflight=16)
+    for i, m in enumerate(meshes):
+        srv.add_mesh(i, m)
+    # warm-up: one small request per mesh loads every mesh into HBM
+    warm = [srv.submit(i, S.MODE_SDF, *[p.array[:1024] for p in pools[i]], outs[0][0].array[:1024],
+                       outs[0][1].array[:1024], flags=S.FLAG_FIXUP) for i in range(len(meshes))]
+    for t in warm:
+        srv.wait(t)
+    slots = [None] * ring
+    lat = []
+    t0 = srv.clock() + 0.05
+    log(f"[bench] serve: {len(meshes)} meshes resident, {len(times)} requests over {args.serve_seconds} s")
+    for k in range(len(times)):
+        if k % 250 == 0:
+            log(f"[bench] serve: request {k} at t={srv.clock() - t0:.2f} s")
+        slot = k % ring
+        if slots[slot] is not None:
+            lat.append(srv.wait(slots[slot]))
+            slots[slot] = None
+        arrival = t0 + float(times[k])
+        delay = arrival - srv.clock()
+        if delay > 0:
+            time.sleep(delay)
+        i, n = int(mesh_ids[k]), int(nq[k])
+        x, y, z = (p.array[:n] for p in pools[i])
+        slots[slot] = srv.submit(i, S.MODE_SDF, x, y, z, outs[slot][0].array[:n], outs[slot][1].array[:n],
+                                 flags=S.FLAG_FIXUP, arrival=arrival)
+    log(f"[bench] serve: all submitted at t={srv.clock() - t0:.2f} s, draining")
+    for tk in slots:
+        if tk is not None:
+            lat.append(srv.wait(tk))
+    t_end = srv.clock()
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    total_q = int(nq.sum())
+    res = {"requests": len(times), "seconds_offered": args.serve_seconds, "meshes": len(meshes),
+           "triangles": int(sum(m.ntris for m in meshes)), "queries": total_q,
+           "offered_mqps": round(total_q / args.serve_seconds / 1e6, 2),
+           "throughput_mqps": round(total_q / (t_end - t0) / 1e6, 2),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+           "max_ms": round(float(lat.max()) * 1e3, 2),
+           "mesh_loads": int(st["mesh_loads"]), "evictions": int(st["evictions"]), "errors": int(st["errors"]),
+           "hbm_budget_gib": budget / 2**30, "peak_gib": round(st["peak_bytes"] / 2**30, 3),
+           "setup_s": round(gen_s, 1)}
+    log(f"[bench] serve: {res}")
+    return res
+
+
 def cpu_baseline(args, m, n, lo, hi):
     """The oracle's edf_celllist<float> restatement (serial/ssdf.hpp:642-963) on a seeded uniform
     subsample of the grid, all host threads."""
@@ -254,6 +332,8 @@ def main():
     ap.add_argument("--only", choices=["fixup", "plain"], default=None,
                     help="time one mode only and print nothing else (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-mesh batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window of the serving leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -288,12 +368,12 @@ def main():
         "host_call_plain": {"value": round(g["host_call_plain_mqps"], 2), "seconds": round(g["host_call_plain_s"], 4)},
         "fixup_queued": g["fixup_queued"], "kernel_error": g["kernel_error"], "lbvh_maxdepth": g["maxdepth"],
         "build_ms": round(g["build_ms"], 3),
-        "served": None,
     }
     if args.grid == 256:
         line["roofline"] = roofline(g["fixup"]["query_ms"], N, m.nverts, m.ntris)
     if not args.no_batch:
         line["batch64"] = run_batch(args)
+    line["served"] = None if args.no_serve else run_serve(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
     print(json.dumps(line), flush=True)

@@ -19,6 +19,8 @@ struct Workspace {
     unsigned *status = nullptr;  // [0] fixup count, [1] kernel error flag
     unsigned *list = nullptr;
     size_t cap = 0;
+    void *sort = nullptr;  // Morton sort of scattered query points
+    size_t sort_cap = 0;
 };
 
 __global__ void k_reset_status(unsigned *status) {
@@ -40,7 +42,18 @@ int ws_reserve(Workspace &w, size_t n, hipStream_t stream) {
 void ws_free(Workspace &w, hipStream_t stream) {
     dev_free(w.status, stream);
     dev_free(w.list, stream);
+    dev_free(w.sort, stream);
     w = Workspace{};
+}
+
+int ws_sort_reserve(Workspace &w, size_t bytes, hipStream_t stream) {
+    if (w.sort_cap >= bytes) return SSDF_OK;
+    dev_free(w.sort, stream);
+    w.sort = nullptr;
+    w.sort_cap = 0;
+    SSDF_HIP_TRY(dev_alloc(&w.sort, bytes, stream));
+    w.sort_cap = bytes;
+    return SSDF_OK;
 }
 
 // Exact FP64 argmin over every triangle (tiles of 256 TriF headers in LDS).
@@ -249,6 +262,12 @@ int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void 
     a.out_tri = out_tri;
     a.out_cx = cx, a.out_cy = cy, a.out_cz = cz;
     const int dbl = h->m.dtype;
+    if (dbl == 0 && n >= kSortMin) {
+        if (int rc = ws_sort_reserve(h->ws, sort_ws_bytes(n), stream)) return rc;
+        if (int rc = sort_points(h->m.meta, n, (const float *)x, (const float *)y, (const float *)z, h->ws.sort,
+                                 &a.perm, stream))
+            return rc;
+    }
     if (int rc = launch_query(h->m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream)) return rc;
     if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags))
         return launch_gwn(h->m, a, QSRC_POINTS, GW_SIGN, dbl, stream);
@@ -290,6 +309,12 @@ int gwn_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y
     a.npoints = n;
     a.out_w = out_w;
     a.out_in = out_in;
+    if (h->m.dtype == 0 && n >= kSortMin) {
+        if (int rc = ws_sort_reserve(h->ws, sort_ws_bytes(n), stream)) return rc;
+        if (int rc = sort_points(h->m.meta, n, (const float *)x, (const float *)y, (const float *)z, h->ws.sort,
+                                 &a.perm, stream))
+            return rc;
+    }
     return launch_gwn(h->m, a, QSRC_POINTS, mode, h->m.dtype, stream);
 }
 

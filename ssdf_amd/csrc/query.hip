@@ -17,6 +17,7 @@
 // k_query_exact: one lane per query, FP64 box and triangle tests, private stack; exact argmin with
 // the canonical tie-break.  Used for FP64 meshes and for the fixup queue.
 #include "query.hpp"
+#include "radix.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -129,6 +130,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     float thr = valid ? init_thr : -SSDF_INF;  // FIX: inclusive bound
     int bk = -1, ncand = 0;
     float bestd = SSDF_INF;  // FIX: FP32 best distance found
+
 
     // Stack of pending subtrees: entry i sits in lane i % 64 of VGPR i / 64 (written by a lane
     // select at the wave-uniform depth sp, read back with v_readlane), each naming where the subtree
@@ -339,11 +341,16 @@ __device__ __forceinline__ void tri_verts64(const QueryArgs &a, const TriF &tr, 
 
 constexpr int kExactStack = 128;
 
+template <typename G, int OUT>
+__device__ void exact_write(const QueryArgs &a, long long qid, double qx, double qy, double qz, double out_init,
+                            double best, int bt, int bk);
+
 // One query, exact FP64: qid of the query source; band2 bounds the search (the FP32 band of a
 // queued fixup query, +inf otherwise).
-template <typename G, int QSRC, int OUT>
-__device__ void exact_one(const QueryArgs &a, const long long qid, const double band2) {
-    double qx, qy, qz;
+// Coordinates of query qid in caller precision G, widened to FP64.
+template <typename G, int QSRC>
+__device__ __forceinline__ void query_coords(const QueryArgs &a, const long long qid, double &qx, double &qy,
+                                             double &qz) {
     if (QSRC == QSRC_POINTS) {
         qx = (double)load_as<G>(a.qx, qid), qy = (double)load_as<G>(a.qy, qid), qz = (double)load_as<G>(a.qz, qid);
     } else {
@@ -353,6 +360,12 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
         qy = (double)((G)a.gmin[1] + (G)j * hy);
         qz = (double)((G)a.gmin[2] + (G)k * hz);
     }
+}
+
+template <typename G, int QSRC, int OUT>
+__device__ void exact_one(const QueryArgs &a, const long long qid, const double band2) {
+    double qx, qy, qz;
+    query_coords<G, QSRC>(a, qid, qx, qy, qz);
     double out_init = __builtin_inf();
     if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
     const double init2 = out_init * out_init;
@@ -421,7 +434,14 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
         }
         if (node == kNone) break;
     }
-    // outputs in caller precision G
+    exact_write<G, OUT>(a, qid, qx, qy, qz, out_init, best, bt, bk);
+}
+
+// Writes the result of an exact search (FP64 best d2, original index bt, leaf position bk; bk < 0:
+// nothing below the caller's bound) in caller precision G.
+template <typename G, int OUT>
+__device__ void exact_write(const QueryArgs &a, const long long qid, const double qx, const double qy,
+                            const double qz, const double out_init, const double best, const int bt, const int bk) {
     const double dist = sqrt(best);
     if (OUT == OUT_EDF) {
         if (bk >= 0) {
@@ -458,6 +478,116 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
                                         a.vertex_n, nl64);
     store_as<G>(a.out_d, qid, (G)(dot < 0 ? -dist : dist));
     if (a.out_tri) a.out_tri[qid] = bt;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Fixup queue, one wave per queued query: the exact FP64 search of exact_one, breadth-first.  The
+// wave expands a frontier of nodes 64 at a time (each lane tests one node's two child boxes in
+// FP64 against the bound and the survivors are compacted into the next frontier by ballots; a
+// leaf is evaluated in FP64 by the lane that holds it), and after every level the bound tightens
+// to the wave's best.  A frontier wider than kFrontier falls back to exact_one.  Same bound, same
+// tie-break (d2, then original index), so the same result as exact_one.
+// ------------------------------------------------------------------------------------------------
+constexpr int kFrontier = 1024;
+
+__device__ __forceinline__ void wave_min3(double &d, int &t, int &k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double od = __shfl_xor(d, off);
+        const int ot = __shfl_xor(t, off), ok = __shfl_xor(k, off);
+        if (od < d || (od == d && ot < t)) d = od, t = ot, k = ok;
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename G, int QSRC, int OUT>
+__device__ bool fix_wave_one(const QueryArgs &a, const long long qid, const double band2, int *cur, int *nxt) {
+    const int lane = threadIdx.x & 63;
+    double qx, qy, qz;
+    query_coords<G, QSRC>(a, qid, qx, qy, qz);
+    double out_init = __builtin_inf();
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
+    const double init2 = out_init * out_init;
+    const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
+    double bound;
+    {
+        const double b0 = sqrt(fmin(init2, band2)) + delta;
+        bound = b0 * b0;
+    }
+    double best = __builtin_inf();
+    int bt = INT_MAX, bk = -1;
+    const int root = uniform_i(a.meta->root);
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int ncur = 1;
+    for (int level = 0; ncur > 0; level++) {
+        if (level > 4096) return false;  // not a tree
+        int nnext = 0;
+        for (int base = 0; base < ncur; base += 64) {
+            const int idx = base + lane;
+            bool p0 = false, p1 = false;
+            int c0 = 0, c1 = 0;
+            if (idx < ncur) {
+                const int node = level == 0 ? root : cur[idx];
+                if (node < 0) {
+                    const TriF tr = a.tris[~node].f;
+                    double A[3], B[3], C[3];
+                    tri_verts64<G>(a, tr, A, B, C);
+                    const double d =
+                        pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
+                    if (d < init2 && (d < best || (d == best && tr.orig < bt))) best = d, bt = tr.orig, bk = ~node;
+                } else {
+                    const Node nd = a.nodes[node];
+                    const double dl = pt_box_d2<double>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
+                                                        nd.lmax[1], nd.lmax[2]);
+                    const double dr = pt_box_d2<double>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
+                                                        nd.rmax[1], nd.rmax[2]);
+                    p0 = dl <= bound, p1 = dr <= bound;
+                    c0 = nd.left, c1 = nd.right;
+                }
+            }
+            const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+            const int pos0 = nnext + __popcll(m0 & below);
+            const int pos1 = nnext + __popcll(m0) + __popcll(m1 & below);
+            if (p0 && pos0 < kFrontier) nxt[pos0] = c0;
+            if (p1 && pos1 < kFrontier) nxt[pos1] = c1;
+            nnext += __popcll(m0) + __popcll(m1);
+        }
+        double wb = best;
+        int wt = bt, wk = bk;
+        wave_min3(wb, wt, wk);
+        if (wb < __builtin_inf()) {
+            const double r = sqrt(wb) + delta;
+            bound = fmin(bound, r * r);
+        }
+        if (nnext > kFrontier) return false;
+        wave_lds_sync();
+        int *t = cur;
+        cur = nxt, nxt = t;
+        ncur = nnext;
+    }
+    wave_min3(best, bt, bk);
+    if (lane == 0) exact_write<G, OUT>(a, qid, qx, qy, qz, out_init, best, bt, bk);
+    return true;
+}
+
+constexpr int kFixWaves = 4;
+
+template <int QSRC, int OUT>
+__global__ __launch_bounds__(64 * kFixWaves) void k_fix_wave(QueryArgs a) {
+    __shared__ int fr[kFixWaves][2][kFrontier];
+    const int w = threadIdx.x >> 6;
+    const unsigned n_items = min(*a.fix_count, a.fix_cap);
+    for (unsigned item = blockIdx.x * kFixWaves + w; item < n_items; item += gridDim.x * kFixWaves) {
+        const unsigned *e = a.fix_list + 4 * (size_t)item;
+        const long long qid = (long long)e[0];
+        const double band2 = (double)__uint_as_float(e[1]);
+        if (!fix_wave_one<float, QSRC, OUT>(a, qid, band2, fr[w][0], fr[w][1]))
+            if ((threadIdx.x & 63) == 0) exact_one<float, QSRC, OUT>(a, qid, band2);
+    }
 }
 
 // QUEUE: the items are the fixup queue's entries (query id, FP32 band); else every query.
@@ -547,9 +677,11 @@ __global__ __launch_bounds__(256) void k_batch_packet(const Node *__restrict__ n
 }
 
 template <int QSRC, int OUT>
-__global__ __launch_bounds__(64) void k_batch_exact(const BatchMesh *__restrict__ meshes, BatchArgs b) {
+__global__ __launch_bounds__(64 * kFixWaves) void k_batch_exact(const BatchMesh *__restrict__ meshes, BatchArgs b) {
+    __shared__ int fr[kFixWaves][2][kFrontier];
+    const int w = threadIdx.x >> 6;
     const unsigned n_items = min(*b.base.fix_count, b.base.fix_cap);
-    for (unsigned item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+    for (unsigned item = blockIdx.x * kFixWaves + w; item < n_items; item += gridDim.x * kFixWaves) {
         const unsigned *e = b.base.fix_list + 4 * (size_t)item;
         const int mi = (int)e[2];
         // descriptors are ordered by unit, not by mesh: find the mesh's entry
@@ -557,7 +689,10 @@ __global__ __launch_bounds__(64) void k_batch_exact(const BatchMesh *__restrict_
         while (j < b.nmeshes - 1 && meshes[j].mesh != mi) j++;
         QueryArgs a;
         batch_args(b, meshes[j], a);
-        exact_one<float, QSRC, OUT>(a, (long long)e[0], (double)__uint_as_float(e[1]));
+        const long long qid = (long long)e[0];
+        const double band2 = (double)__uint_as_float(e[1]);
+        if (!fix_wave_one<float, QSRC, OUT>(a, qid, band2, fr[w][0], fr[w][1]))
+            if ((threadIdx.x & 63) == 0) exact_one<float, QSRC, OUT>(a, qid, band2);
     }
 }
 
@@ -575,7 +710,7 @@ static void launch_batch_packet(const BatchArgs &b, const BatchMesh *meshes, boo
     else
         hipLaunchKernelGGL((k_batch_packet<STACK, QSRC, OUT, false>), dim3(nblocks), dim3(256), 0, s, b.nodes,
                            b.tris, meshes, b);
-    if (fix) hipLaunchKernelGGL((k_batch_exact<QSRC, OUT>), dim3(256), dim3(64), 0, s, meshes, b);
+    if (fix) hipLaunchKernelGGL((k_batch_exact<QSRC, OUT>), dim3(512), dim3(64 * kFixWaves), 0, s, meshes, b);
 }
 
 template <int STACK, int QSRC>
@@ -607,6 +742,56 @@ int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out,
         return rc;
     }
     SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Morton order of scattered query points
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_point_keys(long long n, const float *__restrict__ x,
+                                                    const float *__restrict__ y, const float *__restrict__ z,
+                                                    const MeshMeta *__restrict__ meta, uint32_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ vals) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p[3] = {x[i], y[i], z[i]};
+    uint32_t q[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float lo = meta->root_box[d], hi = meta->root_box[3 + d];
+        const float ext = hi - lo;
+        float u = ext > 0.0f ? (p[d] - lo) / ext : 0.0f;
+        u = fminf(fmaxf(u, 0.0f), 1.0f);  // outside the box (or NaN): clamped
+        q[d] = min((uint32_t)(u * 1024.0f), 1023u);
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = (uint32_t)i;
+}
+
+size_t sort_ws_bytes(long long n) {
+    const size_t nn = (size_t)std::max<long long>(n, 1);
+    return 16 * nn + 4 * radix_scratch_words(nn) + 256;
+}
+
+int sort_points(const MeshMeta *meta, long long n, const float *x, const float *y, const float *z, void *ws,
+                const unsigned **perm, hipStream_t stream) {
+    const size_t nn = (size_t)n;
+    uint32_t *keys = (uint32_t *)ws, *keys_alt = keys + nn, *vals = keys_alt + nn, *vals_alt = vals + nn;
+    uint32_t *scratch = vals_alt + nn;
+    hipLaunchKernelGGL(k_point_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, x, y, z, meta, keys,
+                       vals);
+    uint32_t *sk = nullptr, *sv = nullptr;
+    SSDF_HIP_TRY(radix_sort_pairs<uint32_t>(keys, keys_alt, vals, vals_alt, nn, 30, scratch, stream, &sk, &sv));
+    *perm = sv;
     return SSDF_OK;
 }
 
@@ -703,12 +888,21 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
     }
     if (rc) return rc;
     if (fix) {
-        // queued overflow lanes: exact FP64 traversal, grid-stride over the device-side count
-        const unsigned nb = 1024;
-        if (qsrc == QSRC_POINTS)
-            rc = dispatch_exact_out<float, QSRC_POINTS, true>(a, out, nb, stream);
-        else
-            rc = dispatch_exact_out<float, QSRC_GRID, true>(a, out, nb, stream);
+        // queued queries: one wave each, grid-stride over the device-side count
+        const dim3 g(512), blk(64 * kFixWaves);
+        if (qsrc == QSRC_POINTS) {
+            if (out == OUT_EDF)
+                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_EDF>), g, blk, 0, stream, a);
+            else if (out == OUT_SDF)
+                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_SDF>), g, blk, 0, stream, a);
+            else
+                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_CLOSEST>), g, blk, 0, stream, a);
+        } else {
+            if (out == OUT_EDF)
+                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_EDF>), g, blk, 0, stream, a);
+            else
+                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_SDF>), g, blk, 0, stream, a);
+        }
     }
     SSDF_HIP_TRY(hipGetLastError());
     return rc;

@@ -34,6 +34,7 @@ struct QueryArgs {
     // queries: points (SoA, caller precision) ...
     const void *qx, *qy, *qz;
     long long npoints;
+    const unsigned *perm;  // optional: wave slot j answers point perm[j] (Morton order)
     // ... or grid nodes x_i = gmin + i * gh (x fastest)
     int nx, ny, nz;
     int nbx, nby, nbz;  // 4x4x4 bricks per axis
@@ -128,6 +129,16 @@ struct BatchArgs {
 int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out, bool fix, int stack,
                  unsigned nblocks, hipStream_t stream);
 
+// Scattered points: a wave of 64 consecutive points that lie all over the mesh walks the union of
+// their 64 paths.  For FP32 point sets of at least kSortMin points the queries are ordered by a
+// 30-bit Morton code of their position in the mesh's root box (4 radix passes) and the waves take
+// them through the resulting permutation; results still land at the points' own positions.
+constexpr long long kSortMin = 4096;
+size_t sort_ws_bytes(long long n);
+// perm (in ws) = the Morton order of the n points; ws holds sort_ws_bytes(n) bytes.
+int sort_points(const MeshMeta *meta, long long n, const float *x, const float *y, const float *z, void *ws,
+                const unsigned **perm, hipStream_t stream);
+
 // Generalized winding number of the queries (gwn.hip): GW_WINDING writes w (float) to out_w,
 // GW_INSIDE writes w >= 1/2 (uint8) to out_in, GW_SIGN gives out_d (mesh precision) the sign
 // -|d| where w >= 1/2, +|d| elsewhere.  Builds the mesh's dipole tree on first use.
@@ -157,8 +168,9 @@ template <int QSRC, typename Q>
 __device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id, int lane, long long &qid, Q &x,
                                            Q &y, Q &z) {
     if (QSRC == QSRC_POINTS) {
-        qid = (long long)wave_id * 64 + lane;
-        if (qid >= a.npoints) return false;
+        const long long j = (long long)wave_id * 64 + lane;
+        if (j >= a.npoints) return false;
+        qid = a.perm ? (long long)a.perm[j] : j;  // spatially sorted order of scattered points
         x = load_as<Q>(a.qx, qid), y = load_as<Q>(a.qy, qid), z = load_as<Q>(a.qz, qid);
         return true;
     } else {

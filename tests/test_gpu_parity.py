@@ -316,3 +316,35 @@ def test_moving_mesh_update_vertices(oracle):
         assert np.array_equal(t.download(), tri)
         assert np.abs(o.download() - s).max() <= 1e-6 * 2
     assert rm.info()["open_edges"] == 0
+
+
+def test_fixup_queue_many_ties(oracle):
+    """Points on the axis of a UV sphere are equidistant from every triangle of a latitude ring (48
+    exact ties, beyond the 24-entry candidate list): they take the fixup queue, whose FP64 search must
+    return the oracle's canonical (lowest) index."""
+    m = mg.uv_sphere(0.5, 48, 24)
+    rng = np.random.default_rng(5)
+    zs = np.linspace(-0.3, 0.3, 61)
+    P = np.concatenate([np.stack([np.zeros_like(zs), np.zeros_like(zs), zs], 1),
+                        rng.normal(0, 1e-3, (6000, 3))]).astype(np.float32)
+    x, y, z = (np.ascontiguousarray(P[:, d]) for d in range(3))
+    rm = S.ResidentMesh.from_mesh(m)
+    dx, dy, dz = _dev(x, y, z)
+    out = S.DeviceArray(x.size, np.float32)
+    tri = S.DeviceArray(x.size, np.int32)
+    rm.query_points(S.MODE_SDF, dx, dy, dz, out, tri, flags=S.FLAG_FIXUP)
+    q, err = rm.last_stats()
+    assert err == 0 and q > 61, q
+    s, rtri, _, _, _ = oracle.sdf(x, y, z, m)
+    assert np.array_equal(tri.download(), rtri)
+    assert np.abs(out.download() - s).max() <= 1e-6 * _diag(m)
+    # grid form through the same queue: a grid whose middle node is the centre
+    n = (9, 9, 61)
+    out_g = S.DeviceArray(9 * 9 * 61, np.float32)
+    tri_g = S.DeviceArray(9 * 9 * 61, np.int32)
+    rm.query_grid(S.MODE_SDF, n, (-0.01, -0.01, -0.3), (0.01, 0.01, 0.3), out_g, tri_g, flags=S.FLAG_FIXUP)
+    q, err = rm.last_stats()
+    assert err == 0 and q > 0
+    gx, gy, gz = mg.grid_points(*n, (-0.01, -0.01, -0.3), (0.01, 0.01, 0.3))
+    s, rtri, _, _, _ = oracle.sdf(gx, gy, gz, m)
+    assert np.array_equal(tri_g.download(), rtri)
