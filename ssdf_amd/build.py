@@ -95,10 +95,26 @@ def build_cpp_tests() -> list[Path]:
     return outs
 
 
+def build_tools() -> list[Path]:
+    """Compiles the command-line programs (csrc/*_main.cpp, host code over the C ABI) into _lib/,
+    next to the library they load (rpath $ORIGIN): sdf_main.cpp -> ssdf-sdf."""
+    outs = []
+    for src in sorted(CSRC.glob("*_main.cpp")):
+        exe = LIBDIR / ("ssdf-" + src.stem[: -len("_main")])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", str(PKG.parent / "include"), str(src), "-L", str(LIBDIR),
+               "-lssdf_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src.name}:\n{r.stderr}")
+        outs.append(exe)
+    return outs
+
+
 def clean() -> None:
     shutil.rmtree(OBJDIR, ignore_errors=True)
 
 
 if __name__ == "__main__":
     p = build(verbose="-v" in sys.argv)
+    build_tools()
     print(p)

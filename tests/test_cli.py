@@ -1,0 +1,36 @@
+"""The sdf command (drivers/sdf.exe.cpp contract): argument and file checks that exit before any
+device work run on the CPU; the GPU runs are in test_gpu_cli.py."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from ssdf_amd import meshgen as mg
+from ssdf_amd import rawio
+
+
+def _run(*args, env=None):
+    if not rawio.SDF_EXE.exists():
+        pytest.skip("ssdf-sdf not built")
+    return subprocess.run([str(rawio.SDF_EXE), *map(str, args)], capture_output=True, text=True, env=env,
+                          timeout=120)
+
+
+def test_usage_exits_1():
+    r = _run()
+    assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_missing_and_mismatched_files(tmp_path):
+    m = mg.uv_sphere(0.5, 8, 4)
+    rawio.write_surface(tmp_path / "s", m)
+    rawio.write_points(tmp_path / "p", np.zeros(5), np.zeros(5), np.zeros(5))
+    r = _run(tmp_path / "nope", tmp_path / "p", tmp_path / "o.raw")
+    assert r.returncode == 1 and "Cannot open" in r.stderr
+    np.zeros(3, np.float32).tofile(tmp_path / "s" / "y.float32")
+    r = _run(tmp_path / "s", tmp_path / "p", tmp_path / "o.raw")
+    assert r.returncode == 1 and "mismatched" in r.stderr
+    rawio.write_surface(tmp_path / "s", m)
+    np.zeros(4, np.float32).tofile(tmp_path / "p" / "z.float32")
+    r = _run(tmp_path / "s", tmp_path / "p", tmp_path / "o.raw")
+    assert r.returncode == 1 and "mismatched" in r.stderr
