@@ -271,6 +271,62 @@ def run_serve(args):
     return res
 
 
+def run_beams(args):
+    """200 capped cylinders in the unit cube (3.4 M triangles), intact and with 1% of the triangles
+    deleted; signed distance + triangle on the 512^3 grid over the cube (FP32 + fixup; the holed
+    mesh signed by winding numbers through SSDF_FLAG_SIGN_AUTO) and at 1e8 uniform scattered points."""
+    import ssdf_amd as S
+    from ssdf_amd import workloads as W
+    t0 = time.perf_counter()
+    full = W.beams()
+    holed = W.beams(hole_fraction=0.01)
+    log(f"[bench] beams: meshes generated ({time.perf_counter() - t0:.1f} s)")
+    g = args.beams_grid
+    N = g ** 3
+    st = S.Stream()
+    ev = [S.Event(), S.Event()]
+    res = {"triangles": full.ntris, "triangles_holed": holed.ntris, "grid": g}
+
+    def timed(f):
+        f()
+        st.synchronize()
+        ev[0].record(st)
+        f()
+        ev[1].record(st)
+        st.synchronize()
+        return ev[0].elapsed_ms(ev[1])
+
+    outs = {}
+    for key, m, flags in (("closed", full, S.FLAG_FIXUP), ("holed_gwn", holed, S.FLAG_FIXUP | S.FLAG_SIGN_AUTO)):
+        rm = S.ResidentMesh.from_mesh(m, stream=st)
+        o = S.DeviceArray(N, np.float32)
+        t = S.DeviceArray(N, np.int32)
+        ms = timed(lambda: rm.query_grid(S.MODE_SDF, (g, g, g), (0, 0, 0), (1, 1, 1), o, t, flags=flags, stream=st))
+        q, e = rm.last_stats()
+        res[key] = {"grid_ms": round(ms, 2), "grid_mqps": round(N / ms / 1e3, 1), "fixup_queued": q,
+                    "kernel_error": e, "open_edges": rm.info()["open_edges"]}
+        outs[key] = o.download()
+        log(f"[bench] beams {key}: grid {ms:.1f} ms")
+        if key == "closed":
+            n = args.beams_points
+            P = np.random.default_rng(5).random((3, n), dtype=np.float32)
+            dx, dy, dz = (S.DeviceArray.from_host(P[d]) for d in range(3))
+            del P
+            po = S.DeviceArray(n, np.float32)
+            pt = S.DeviceArray(n, np.int32)
+            ms = timed(lambda: rm.query_points(S.MODE_SDF, dx, dy, dz, po, pt, flags=flags, stream=st))
+            q, e = rm.last_stats()
+            res["scattered"] = {"points": n, "ms": round(ms, 2), "mqps": round(n / ms / 1e3, 1), "fixup_queued": q,
+                                "kernel_error": e}
+            log(f"[bench] beams scattered: {ms:.1f} ms")
+            del dx, dy, dz, po, pt
+        del rm, o, t
+    a, b = outs["closed"], outs["holed_gwn"]
+    away = np.abs(a) > 0.01
+    res["holed_sign_agreement_beyond_0.01"] = float(np.mean(np.sign(a[away]) == np.sign(b[away])))
+    return res
+
+
 def cpu_baseline(args, m, n, lo, hi):
     """The oracle's edf_celllist<float> restatement (serial/ssdf.hpp:642-963) on a seeded uniform
     subsample of the grid, all host threads."""
@@ -333,6 +389,9 @@ def main():
                     help="time one mode only and print nothing else (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-mesh batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-beams", action="store_true", help="skip the 200-beam stress leg")
+    ap.add_argument("--beams-grid", type=int, default=512)
+    ap.add_argument("--beams-points", type=int, default=100_000_000)
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window of the serving leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (traffic-counter calibration)")
@@ -374,6 +433,8 @@ def main():
     if not args.no_batch:
         line["batch64"] = run_batch(args)
     line["served"] = None if args.no_serve else run_serve(args)
+    if not args.no_beams:
+        line["beams200"] = run_beams(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
     print(json.dumps(line), flush=True)

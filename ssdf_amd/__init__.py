@@ -424,8 +424,9 @@ class ResidentMesh:
         self.h = h.value
 
     @classmethod
-    def from_mesh(cls, m, with_normals=True, dtype=np.float32):
-        return cls(m.s0, m.s1, m.s2, m.sx.astype(dtype), m.sy.astype(dtype), m.sz.astype(dtype), with_normals)
+    def from_mesh(cls, m, with_normals=True, dtype=np.float32, stream: Stream | None = None):
+        return cls(m.s0, m.s1, m.s2, m.sx.astype(dtype), m.sy.astype(dtype), m.sz.astype(dtype), with_normals,
+                   stream)
 
     def info(self) -> dict:
         i = _MeshInfo()

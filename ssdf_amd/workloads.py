@@ -88,3 +88,85 @@ def poisson_requests(rate: float, duration: float, nmeshes: int, seed: int = 11,
     mesh = (rng.uniform(k) * nmeshes).astype(np.int64) % nmeshes
     nq = np.exp(math.log(qmin) + rng.uniform(k) * (math.log(qmax) - math.log(qmin))).astype(np.int64)
     return np.array(times), mesh, nq
+
+
+def _seg_seg_dist(p0, p1, q0, q1) -> float:
+    """Distance between segments [p0, p1] and [q0, q1] (closest points by clamped parameters)."""
+    d1, d2, r = p1 - p0, q1 - q0, p0 - q0
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    c, b = d1 @ r, d1 @ d2
+    den = a * e - b * b
+    s = np.clip((b * f - c * e) / den, 0.0, 1.0) if den > 1e-18 else 0.0
+    t = (b * s + f) / e
+    if t < 0.0:
+        t, s = 0.0, np.clip(-c / a, 0.0, 1.0)
+    elif t > 1.0:
+        t, s = 1.0, np.clip((b - c) / a, 0.0, 1.0)
+    return float(np.linalg.norm((p0 + d1 * s) - (q0 + d2 * t)))
+
+
+def capped_cylinder(p0, p1, r: float, n_circ: int, n_axial: int):
+    """Closed, outward-oriented cylinder of radius r around the segment p0 -> p1: n_axial rings of
+    n_circ quads (two triangles each) and a triangle fan on each end: V = n_circ (n_axial + 1) + 2,
+    T = 2 n_circ (n_axial + 1)."""
+    axis = p1 - p0
+    L = float(np.linalg.norm(axis))
+    w = axis / L
+    u = np.cross(w, [1.0, 0.0, 0.0] if abs(w[0]) < 0.9 else [0.0, 1.0, 0.0])
+    u /= np.linalg.norm(u)
+    v = np.cross(w, u)
+    th = 2.0 * np.pi * np.arange(n_circ) / n_circ
+    ring = np.cos(th)[:, None] * u + np.sin(th)[:, None] * v  # outward unit directions
+    t = np.arange(n_axial + 1) / n_axial
+    verts = (p0 + t[:, None, None] * axis + r * ring[None, :, :]).reshape(-1, 3)
+    verts = np.vstack([verts, p0, p1])
+    c0, c1 = verts.shape[0] - 2, verts.shape[0] - 1
+    i = np.arange(n_axial)[:, None]
+    j = np.arange(n_circ)[None, :]
+    a = i * n_circ + j
+    b = i * n_circ + (j + 1) % n_circ
+    c = (i + 1) * n_circ + (j + 1) % n_circ
+    d = (i + 1) * n_circ + j
+    side = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    jj = np.arange(n_circ)
+    cap0 = np.stack([np.full(n_circ, c0), (jj + 1) % n_circ, jj], 1)  # facing -w
+    top = n_axial * n_circ
+    cap1 = np.stack([np.full(n_circ, c1), top + jj, top + (jj + 1) % n_circ], 1)  # facing +w
+    return verts, np.concatenate([side, cap0, cap1])
+
+
+def beams(nbeams: int = 200, seed: int = 17, n_circ: int = 64, axial_per_half_unit: int = 194,
+          hole_fraction: float = 0.0):
+    """Capped cylinders with radius U[0.01, 0.03], length U[0.2, 0.8] and uniformly random
+    orientation, kept only if their axes stay further apart than the sum of the radii (plus 0.005)
+    and inside the unit cube; about 25 k triangles each.  hole_fraction > 0 deletes that seeded
+    fraction of the triangles (an open surface)."""
+    rng = np.random.default_rng(seed)
+    segs, verts, faces = [], [], []
+    off = 0
+    tries = 0
+    while len(segs) < nbeams:
+        tries += 1
+        if tries > 200 * nbeams:
+            raise RuntimeError("beams: could not place the cylinders")
+        r = rng.uniform(0.01, 0.03)
+        L = rng.uniform(0.2, 0.8)
+        w = rng.normal(size=3)
+        w /= np.linalg.norm(w)
+        c = rng.uniform(0.0, 1.0, 3)
+        p0, p1 = c - 0.5 * L * w, c + 0.5 * L * w
+        if np.any(np.minimum(p0, p1) < r) or np.any(np.maximum(p0, p1) > 1.0 - r):
+            continue
+        if any(_seg_seg_dist(p0, p1, q0, q1) <= r + s + 0.005 for q0, q1, s in segs):
+            continue
+        segs.append((p0, p1, r))
+        v, f = capped_cylinder(p0, p1, r, n_circ, max(4, int(round(axial_per_half_unit * L / 0.5))))
+        verts.append(v)
+        faces.append(f + off)
+        off += v.shape[0]
+    V = np.concatenate(verts)
+    F = np.concatenate(faces)
+    if hole_fraction > 0:
+        keep = np.random.default_rng(seed + 1).random(F.shape[0]) >= hole_fraction
+        F = F[keep]
+    return mg._mesh(V, F, f"beams_{nbeams}_s{seed}_h{hole_fraction}")
