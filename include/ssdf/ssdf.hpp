@@ -169,6 +169,34 @@ int sample_sdf_bvh(const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0, 
 }
 
 namespace detail {
+inline int sqedf_impl(ptrdiff_t ns, const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const float *sx,
+                      const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, float x0, float y0,
+                      float z0, float x1, float y1, float z1, float *out) {
+    return ssdf_sample_sqedf_f(ns, s0, s1, s2, nv, sx, sy, sz, nx, ny, nz, x0, y0, z0, x1, y1, z1, out,
+                               SSDF_FLAG_FIXUP);
+}
+inline int sqedf_impl(ptrdiff_t ns, const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const double *sx,
+                      const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double x0,
+                      double y0, double z0, double x1, double y1, double z1, double *out) {
+    return ssdf_sample_sqedf_d(ns, s0, s1, s2, nv, sx, sy, sz, nx, ny, nz, x0, y0, z0, x1, y1, z1, out);
+}
+}  // namespace detail
+
+/// serial/bvh/bvh.hpp:183-201: squared unsigned distance on grid nodes, out in/out (min-accumulated).
+template <typename G, typename T, typename I>
+int sample_sqedf_bvh(const ptrdiff_t nselements, const I *const SSDF_RESTRICT s0, const I *const SSDF_RESTRICT s1,
+                     const I *const SSDF_RESTRICT s2, const ptrdiff_t nspoints, const G *const SSDF_RESTRICT sx,
+                     const G *const SSDF_RESTRICT sy, const G *const SSDF_RESTRICT sz, const ptrdiff_t nx,
+                     const ptrdiff_t ny, const ptrdiff_t nz, const G xmin, const G ymin, const G zmin, const G xmax,
+                     const G ymax, const G zmax, T *const SSDF_RESTRICT out) {
+    static_assert(std::is_same<G, T>::value, "ssdf_amd: sample_sqedf_bvh is instantiated for <float,float> and "
+                                             "<double,double>");
+    (void)sizeof(detail::check_types<G, T, I>);
+    return detail::sqedf_impl(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax, ymax,
+                              zmax, out);
+}
+
+namespace detail {
 inline int inside_points_impl(ptrdiff_t n, const float *x, const float *y, const float *z, ptrdiff_t ns,
                               const int *s0, const int *s1, const int *s2, ptrdiff_t nv, const float *sx,
                               const float *sy, const float *sz, uint8_t *out) {

@@ -41,6 +41,7 @@ enum {
 #define SSDF_MODE_EDF 0     /* unsigned distance, min-accumulated into out_d */
 #define SSDF_MODE_SDF 1     /* signed distance (pseudonormal sign) + closest triangle, overwriting */
 #define SSDF_MODE_CLOSEST 2 /* distance + closest point + triangle, on strict improvement only */
+#define SSDF_MODE_SQEDF 3   /* squared unsigned distance, min-accumulated into out_d */
 
 /* ---------------------------------------------------------------------------------------------
  * Reference symbols (host pointers, synchronous; results in host memory on return).
@@ -91,6 +92,19 @@ int ssdf_sample_sdf_d(ptrdiff_t nselements, const int *s0, const int *s1, const 
 int ssdf_sdf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
                const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
                const float *sz, float *out_signed, int *out_tri, unsigned flags);
+
+/* sample_sqedf_bvh (serial/bvh/bvh.hpp:183-201, bvh.cpp:1575-1639): on the nodes of an nx x ny x nz
+ * grid over [min, max] (x fastest), out holds squared distances and becomes min(out, d^2) (callers
+ * seed it, e.g. with FLT_MAX).  The float variant takes query flags (SSDF_FLAG_FIXUP); the double
+ * variant is exact FP64. */
+int ssdf_sample_sqedf_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                        const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                        float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, float *out,
+                        unsigned flags);
+int ssdf_sample_sqedf_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                        const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                        ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                        double *out);
 
 /* points_inside_bvh (serial/bvh/bvh.hpp:53-66, bvh.cpp:318-365): out[i] = 1 if point i is inside
  * the surface, else 0.  The reference decides by cuBQL's ray crossing parity; this engine by the
@@ -179,7 +193,8 @@ int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const 
                            const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy, void *out_cz,
                            unsigned flags, void *stream);
 
-/* Grid nodes x_i = min + i (max - min) / (n - 1), computed in the mesh precision. */
+/* Grid nodes x_i = min + i (max - min) / (n - 1), computed in the mesh precision.  Modes EDF, SDF
+ * or SQEDF. */
 int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,
                          double ymin, double zmin, double xmax, double ymax, double zmax, void *out_d, int *out_tri,
                          unsigned flags, void *stream);

@@ -23,14 +23,14 @@ __all__ = [
     "sample_sdf_bvh", "sdf", "ResidentMesh", "DeviceArray", "Stream", "Event", "FLAG_FIXUP", "FLAG_SIGN_FP64",
     "FLAG_SIGN_GWN", "FLAG_SIGN_AUTO", "points_inside_bvh", "sample_inside_bvh", "compute_aabb",
     "all_points_aabb_signed_distance", "init_volume_box",
-    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST", "Batch", "Server", "PinnedArray",
+    "MODE_EDF", "MODE_SDF", "MODE_CLOSEST", "MODE_SQEDF", "sample_sqedf_bvh", "Batch", "Server", "PinnedArray",
 ]
 
 FLAG_FIXUP = 1
 FLAG_SIGN_FP64 = 2
 FLAG_SIGN_GWN = 8
 FLAG_SIGN_AUTO = 16
-MODE_EDF, MODE_SDF, MODE_CLOSEST = 0, 1, 2
+MODE_EDF, MODE_SDF, MODE_CLOSEST, MODE_SQEDF = 0, 1, 2, 3
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libssdf_amd.so"
 _lib = None
@@ -68,6 +68,8 @@ _SIGS = {
     "ssdf_sample_sdf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double,
                           C.c_double, C.c_double, C.c_double, _P, _P],
     "ssdf_sdf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, C.c_uint],
+    "ssdf_sample_sqedf_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD] + [C.c_float] * 6 + [_P, C.c_uint],
+    "ssdf_sample_sqedf_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD] + [C.c_double] * 6 + [_P],
     "ssdf_points_inside_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
     "ssdf_points_inside_d": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P],
     "ssdf_sample_inside_f": [_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _PD, _PD] + [C.c_float] * 6 + [_P],
@@ -337,6 +339,24 @@ def sdf(x, y, z, s0, s1, s2, sx, sy, sz, out_signed, out_tri=None, flags: int = 
     rc = lib().ssdf_sdf_f(x.size, _ptr(x), _ptr(y), _ptr(z), s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size,
                           _ptr(sx), _ptr(sy), _ptr(sz), _ptr(out_signed), _ptr(out_tri), flags)
     _check(rc, "ssdf_sdf_f")
+    return 0
+
+
+def sample_sqedf_bvh(s0, s1, s2, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, out,
+                     flags: int = 0) -> int:
+    """ssdf::sample_sqedf_bvh (serial/bvh/bvh.hpp:183-201): out (squared distances, in/out) =
+    min(out, d^2) on grid nodes, x fastest."""
+    p = _prec(sx, sy, sz, out)
+    _idx(s0, s1, s2)
+    if out.size != nx * ny * nz:
+        raise ValueError("out must hold nx * ny * nz values")
+    if p == "f":
+        rc = lib().ssdf_sample_sqedf_f(s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy),
+                                       _ptr(sz), nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, _ptr(out), flags)
+    else:
+        rc = lib().ssdf_sample_sqedf_d(s0.size, _ptr(s0), _ptr(s1), _ptr(s2), sx.size, _ptr(sx), _ptr(sy),
+                                       _ptr(sz), nx, ny, nz, xmin, ymin, zmin, xmax, ymax, zmax, _ptr(out))
+    _check(rc, f"ssdf_sample_sqedf_{p}")
     return 0
 
 

@@ -436,6 +436,34 @@ int host_grid_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, 
     return SSDF_OK;
 }
 
+// sample_sqedf_bvh (serial/bvh/bvh.cpp:1575-1639): out (in/out, squared distances) = min(out, d^2)
+// on the grid nodes; synchronous.
+template <typename G>
+int host_sqedf_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, ptrdiff_t nsp, const G *sx,
+                    const G *sy, const G *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, G xmin, G ymin, G zmin,
+                    G xmax, G ymax, G zmax, G *out, unsigned flags) {
+    if (nsel == 0 || nsp == 0 || nx <= 0 || ny <= 0 || nz <= 0) return SSDF_OK;
+    if (!out) {
+        set_error("sample_sqedf: null output array");
+        return SSDF_ERR_INVALID;
+    }
+    const ptrdiff_t nq = nx * ny * nz;
+    hipStream_t stream = nullptr;
+    ssdf_mesh *h = nullptr;
+    const int dtype = sizeof(G) == sizeof(double) ? 1 : 0;
+    if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
+    struct Guard {
+        ssdf_mesh *h;
+        ~Guard() { destroy_mesh(h); }
+    } guard{h};
+    DevBuf dd;
+    if (int rc = h2d(dd, out, nq)) return rc;
+    const double lo[3] = {(double)xmin, (double)ymin, (double)zmin}, hi[3] = {(double)xmax, (double)ymax, (double)zmax};
+    if (int rc = query_grid(h, SSDF_MODE_SQEDF, nx, ny, nz, lo, hi, dd.p, nullptr, flags, stream)) return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(stream));
+    return d2h(out, dd, nq);
+}
+
 // points_inside_bvh (serial/bvh/bvh.cpp:318-365): out[i] = 1 inside, 0 outside; synchronous.
 template <typename G>
 int host_inside_points(ptrdiff_t npoints, const G *x, const G *y, const G *z, ptrdiff_t nsel, const int *s0,
@@ -516,6 +544,22 @@ int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t ns
 }  // namespace ssdf_amd
 
 extern "C" {
+
+int ssdf_sample_sqedf_f(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                        const float *sx, const float *sy, const float *sz, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
+                        float xmin, float ymin, float zmin, float xmax, float ymax, float zmax, float *out,
+                        unsigned flags) {
+    return host_sqedf_call<float>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                  ymax, zmax, out, flags);
+}
+
+int ssdf_sample_sqedf_d(ptrdiff_t nselements, const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints,
+                        const double *sx, const double *sy, const double *sz, ptrdiff_t nx, ptrdiff_t ny,
+                        ptrdiff_t nz, double xmin, double ymin, double zmin, double xmax, double ymax, double zmax,
+                        double *out) {
+    return host_sqedf_call<double>(nselements, s0, s1, s2, nspoints, sx, sy, sz, nx, ny, nz, xmin, ymin, zmin, xmax,
+                                   ymax, zmax, out, 0u);
+}
 
 int ssdf_points_inside_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
                          const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx,
@@ -706,7 +750,7 @@ int ssdf_mesh_info(ssdf_mesh_t mesh, ssdf_mesh_info_t *info) {
 int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const void *x, const void *y,
                            const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy, void *out_cz,
                            unsigned flags, void *stream) {
-    if (!mesh || mode < 0 || mode > 2) {
+    if (!mesh || mode < 0 || mode > 3) {
         set_error("ssdf_mesh_query_points: invalid arguments");
         return SSDF_ERR_INVALID;
     }
@@ -717,7 +761,7 @@ int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const 
 int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,
                          double ymin, double zmin, double xmax, double ymax, double zmax, void *out_d, int *out_tri,
                          unsigned flags, void *stream) {
-    if (!mesh || (mode != SSDF_MODE_SDF && mode != SSDF_MODE_EDF)) {
+    if (!mesh || (mode != SSDF_MODE_SDF && mode != SSDF_MODE_EDF && mode != SSDF_MODE_SQEDF)) {
         set_error("ssdf_mesh_query_grid: invalid arguments");
         return SSDF_ERR_INVALID;
     }

@@ -125,6 +125,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         out_init = ((const float *)a.out_d)[qid];
         init2 = out_init * out_init;
     }
+    if (OUT == OUT_SQEDF && valid) init2 = ((const float *)a.out_d)[qid];  // the seed is already squared
     float best = valid ? init2 : -SSDF_INF;  // plain: strict bound (invalid lanes never pass)
     const float init_thr = __builtin_bit_cast(float, __builtin_bit_cast(int, init2) - 1);  // largest below init2
     float thr = valid ? init_thr : -SSDF_INF;  // FIX: inclusive bound
@@ -257,6 +258,13 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         }
     }
 
+    if (OUT == OUT_SQEDF) {
+        if (k_final >= 0) {
+            ((float *)a.out_d)[qid] = fminf(init2, have64 ? (float)d2_64 : bestd);
+            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
+        }
+        return;
+    }
     const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(bestd);
     if (OUT == OUT_EDF) {
         if (k_final >= 0) {
@@ -367,8 +375,8 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
     double qx, qy, qz;
     query_coords<G, QSRC>(a, qid, qx, qy, qz);
     double out_init = __builtin_inf();
-    if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
-    const double init2 = out_init * out_init;
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST || OUT == OUT_SQEDF) out_init = (double)load_as<G>(a.out_d, qid);
+    const double init2 = OUT == OUT_SQEDF ? out_init : out_init * out_init;  // SQEDF seeds are squared
     double best = __builtin_inf(), bound = __builtin_inf();
     int bt = INT_MAX, bk = -1;
     const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
@@ -442,6 +450,13 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
 template <typename G, int OUT>
 __device__ void exact_write(const QueryArgs &a, const long long qid, const double qx, const double qy,
                             const double qz, const double out_init, const double best, const int bt, const int bk) {
+    if (OUT == OUT_SQEDF) {  // out_init holds the squared seed
+        if (bk >= 0) {
+            store_as<G>(a.out_d, qid, (G)fmin(out_init, best));
+            if (a.out_tri) a.out_tri[qid] = bt;
+        }
+        return;
+    }
     const double dist = sqrt(best);
     if (OUT == OUT_EDF) {
         if (bk >= 0) {
@@ -510,8 +525,8 @@ __device__ bool fix_wave_one(const QueryArgs &a, const long long qid, const doub
     double qx, qy, qz;
     query_coords<G, QSRC>(a, qid, qx, qy, qz);
     double out_init = __builtin_inf();
-    if (OUT == OUT_EDF || OUT == OUT_CLOSEST) out_init = (double)load_as<G>(a.out_d, qid);
-    const double init2 = out_init * out_init;
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST || OUT == OUT_SQEDF) out_init = (double)load_as<G>(a.out_d, qid);
+    const double init2 = OUT == OUT_SQEDF ? out_init : out_init * out_init;  // SQEDF seeds are squared
     const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
     double bound;
     {
@@ -720,6 +735,8 @@ static int launch_batch_out(const BatchArgs &b, const BatchMesh *meshes, int out
         launch_batch_packet<STACK, QSRC, OUT_EDF>(b, meshes, fix, nblocks, s);
     else if (out == OUT_SDF)
         launch_batch_packet<STACK, QSRC, OUT_SDF>(b, meshes, fix, nblocks, s);
+    else if (out == OUT_SQEDF)
+        launch_batch_packet<STACK, QSRC, OUT_SQEDF>(b, meshes, fix, nblocks, s);
     else if (QSRC == QSRC_POINTS)
         launch_batch_packet<STACK, QSRC_POINTS, OUT_CLOSEST>(b, meshes, fix, nblocks, s);
     else
@@ -812,6 +829,9 @@ static int dispatch_packet_out(QueryArgs &a, int out, bool fix, unsigned nblocks
     } else if (out == OUT_SDF) {
         fix ? launch_packet<STACK, QSRC, OUT_SDF, true>(a, nblocks, s)
             : launch_packet<STACK, QSRC, OUT_SDF, false>(a, nblocks, s);
+    } else if (out == OUT_SQEDF) {
+        fix ? launch_packet<STACK, QSRC, OUT_SQEDF, true>(a, nblocks, s)
+            : launch_packet<STACK, QSRC, OUT_SQEDF, false>(a, nblocks, s);
     } else {
         if (QSRC != QSRC_POINTS) return SSDF_ERR_UNSUPPORTED;
         fix ? launch_packet<STACK, QSRC_POINTS, OUT_CLOSEST, true>(a, nblocks, s)
@@ -826,6 +846,8 @@ static int dispatch_exact_out(QueryArgs &a, int out, unsigned nblocks, hipStream
         hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_EDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
     else if (out == OUT_SDF)
         hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+    else if (out == OUT_SQEDF)
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SQEDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
     else
         hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_CLOSEST, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
     return SSDF_OK;
@@ -895,11 +917,15 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
                 hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_EDF>), g, blk, 0, stream, a);
             else if (out == OUT_SDF)
                 hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_SDF>), g, blk, 0, stream, a);
+            else if (out == OUT_SQEDF)
+                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_SQEDF>), g, blk, 0, stream, a);
             else
                 hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_CLOSEST>), g, blk, 0, stream, a);
         } else {
             if (out == OUT_EDF)
                 hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_EDF>), g, blk, 0, stream, a);
+            else if (out == OUT_SQEDF)
+                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_SQEDF>), g, blk, 0, stream, a);
             else
                 hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_SDF>), g, blk, 0, stream, a);
         }

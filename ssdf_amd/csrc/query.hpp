@@ -14,7 +14,8 @@ enum QuerySource : int { QSRC_POINTS = 0, QSRC_GRID = 1 };
 // SDF: signed distance + closest triangle, overwriting (serial/bvh/bvh.cpp:1641-1798 contract).
 // CLOSEST: distance, closest point and triangle, updated only on strict improvement of out_d
 //          (serial/bvh/bvh.cpp:125-197 contract).
-enum QueryOut : int { OUT_EDF = 0, OUT_SDF = 1, OUT_CLOSEST = 2 };
+// SQEDF: squared unsigned distance, min-accumulated into out_d (serial/bvh/bvh.cpp:1575-1639 contract).
+enum QueryOut : int { OUT_EDF = 0, OUT_SDF = 1, OUT_CLOSEST = 2, OUT_SQEDF = 3 };
 
 // flags of the C ABI (ssdf_c.h)
 enum : unsigned {

@@ -348,3 +348,28 @@ def test_fixup_queue_many_ties(oracle):
     gx, gy, gz = mg.grid_points(*n, (-0.01, -0.01, -0.3), (0.01, 0.01, 0.3))
     s, rtri, _, _, _ = oracle.sdf(gx, gy, gz, m)
     assert np.array_equal(tri_g.download(), rtri)
+
+
+def test_sample_sqedf(oracle):
+    """sample_sqedf_bvh (serial/bvh/bvh.cpp:1575-1639): out holds squared distances in/out; FP64 is the
+    exact FP64 argmin's d2, FP32 + fixup within 2 * 1e-6 * diag * d of it; seeds below d^2 stay."""
+    m = mg.torus(0.6, 0.2, 60, 24, sigma=0.004, seed=8)
+    n = (21, 17, 13)
+    lo, hi = (-1.0, -1.0, -0.5), (1.0, 1.0, 0.5)
+    x, y, z = mg.grid_points(*n, lo, hi)
+    d2, _, _, _ = oracle.brute_argmin(x, y, z, m)
+    seed = np.where(np.arange(x.size) % 5 == 0, 0.001, np.finfo(np.float32).max)
+    md = m.as_dtype(np.float64)
+    gx, gy, gz = mg.grid_points(*n, lo, hi, dtype=np.float64)
+    d2d, _, _, _ = oracle.brute_argmin(gx, gy, gz, md)
+    out = seed.astype(np.float64).copy()
+    S.sample_sqedf_bvh(md.s0, md.s1, md.s2, md.sx, md.sy, md.sz, *n, *lo, *hi, out)
+    assert np.array_equal(out, np.minimum(seed, d2d))
+    outf = seed.astype(np.float32).copy()
+    S.sample_sqedf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, *n, *(np.float32(v) for v in lo),
+                       *(np.float32(v) for v in hi), outf, flags=S.FLAG_FIXUP)
+    want = np.minimum(seed, d2)
+    tol = 2e-6 * _diag(m) * np.sqrt(want) + 1e-12
+    assert np.all(np.abs(outf - want) <= tol)
+    stay = (seed < 1) & (d2 > 0.0011)  # seeds clearly below d^2 are kept bit for bit
+    assert stay.sum() > 100 and np.all(outf[stay] == np.float32(0.001))
