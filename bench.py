@@ -385,8 +385,8 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1 << 23)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fixup", "plain"], default=None,
-                    help="time one mode only and print nothing else (for profiling)")
+    ap.add_argument("--only", choices=["fixup", "plain", "batch"], default=None,
+                    help="time one mode (or the 64-mesh batch leg) only and print nothing else (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-mesh batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-beams", action="store_true", help="skip the 200-beam stress leg")
@@ -400,6 +400,9 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (queries shard by "
               f"replicating the mesh and splitting the queries)", file=sys.stderr)
         sys.exit(2)
+    if args.only == "batch":
+        log(json.dumps(run_batch(args)))
+        return
     m, n, lo, hi = make_workload(args)
     N = n[0] * n[1] * n[2]
     log(f"[bench] mesh {m.name}: {m.ntris} triangles; grid {n}; steps {args.steps} warmup {args.warmup}")

@@ -322,6 +322,14 @@ int ssdf_event_elapsed_ms(void *start, void *stop, float *ms);
 int ssdf_mesh_brute_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const float *x, const float *y, const float *z,
                            double *out_d2, int *out_tri, void *stream);
 
+/* Leaf budget of a packet wave (no counterpart in the reference; a tuning knob of this library).
+ * A wave of 64 queries that has evaluated this many leaves hands its unfinished queries to a
+ * per-query kernel (one wave per query) with the bound found so far; results are unchanged (FP32 +
+ * fixup: bit-identical; plain FP32: the same distances, ties may name another triangle).
+ * leaves >= 0 sets it process-wide (0: never hand over) and returns the previous value; leaves < 0
+ * only returns the current one.  Initial value 8192, or SSDF_HEAVY_LEAVES from the environment. */
+int ssdf_set_heavy_leaves(int leaves);
+
 const char *ssdf_last_error(void);
 const char *ssdf_version(void);
 

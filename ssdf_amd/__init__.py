@@ -128,6 +128,7 @@ _SIGS = {
     "ssdf_event_destroy": [_P],
     "ssdf_event_record": [_P, _P],
     "ssdf_event_elapsed_ms": [_P, _P, C.POINTER(C.c_float)],
+    "ssdf_set_heavy_leaves": [C.c_int],
     "ssdf_last_error": [],
     "ssdf_version": [],
 }
@@ -163,6 +164,12 @@ def available() -> bool:
         return n.value > 0
     except (ImportError, OSError):
         return False
+
+
+def set_heavy_leaves(leaves: int) -> int:
+    """Leaf budget of a packet wave before its queries go to the per-query kernel (0: never);
+    process-wide, returns the previous value.  A negative value only reads it."""
+    return int(lib().ssdf_set_heavy_leaves(int(leaves)))
 
 
 def _ptr(a) -> C.c_void_p | None:

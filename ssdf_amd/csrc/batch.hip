@@ -207,7 +207,7 @@ int stage(ssdf_batch *b, const Plan &p, int qsrc, int mode, unsigned flags, cons
     if (!b->queue) SSDF_HIP_TRY(dev_alloc((void **)&b->queue, sizeof(unsigned) * 8 * 64, b->stream));
     if (!b->status) SSDF_HIP_TRY(dev_alloc((void **)&b->status, 64, b->stream));
     const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
-    const size_t entries = fix ? std::min(p.total_queries, std::max<size_t>(1u << 16, p.total_queries / 8)) : 1;
+    const size_t entries = std::min(p.total_queries, std::max<size_t>(1u << 16, p.total_queries / 8));
     if (int rc = ensure_fix(b, std::max<size_t>(entries, 1))) return rc;
     if (st != b->stream) {
         // the buffers above are ordered on the creation stream
@@ -231,6 +231,9 @@ int stage(ssdf_batch *b, const Plan &p, int qsrc, int mode, unsigned flags, cons
     long long maxsel = 1;
     for (auto &m : b->meshes) maxsel = std::max<long long>(maxsel, m.nsel);
     a.base.max_iters = 4 * maxsel + 64;
+    a.base.heavy_iters = heavy_leaves();
+    a.base.heavy_cap = a.base.fix_cap / 2;
+    a.nlev = std::max(b->maxdepth, 1);
     a.nodes = b->nodes, a.tris = b->tris, a.metas = b->metas;
     a.s0 = b->s0, a.s1 = b->s1, a.s2 = b->s2;
     a.sx = b->sx, a.sy = b->sy, a.sz = b->sz;

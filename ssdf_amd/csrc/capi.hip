@@ -2,8 +2,10 @@
 #include "query.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -16,7 +18,7 @@ namespace {
 // 32 ulps of 1.0 (a few roundings of the projection and of p - q, with ample margin).
 
 struct Workspace {
-    unsigned *status = nullptr;  // [0] fixup count, [1] kernel error flag
+    unsigned *status = nullptr;  // [0] queue count, [1] kernel error flag, [2] queue read position
     unsigned *list = nullptr;
     size_t cap = 0;
     void *sort = nullptr;  // Morton sort of scattered query points
@@ -24,7 +26,7 @@ struct Workspace {
 };
 
 __global__ void k_reset_status(unsigned *status) {
-    if (threadIdx.x < 2) status[threadIdx.x] = 0u;
+    if (threadIdx.x < 4) status[threadIdx.x] = 0u;
 }
 
 int ws_reserve(Workspace &w, size_t n, hipStream_t stream) {
@@ -133,10 +135,29 @@ struct ssdf_mesh {
 
 namespace {
 
-// Fixup queue entries reserved for nq queries: near-ties that overflow the candidate list are rare
-// (none in 16.7 M queries of a noisy 200 k-triangle torus), so an eighth of the queries, at least
-// 64 Ki; past that the FP32 result stands and the kernel error flag gets bit 4.
+// Queue entries reserved for nq queries (FIX near-ties that overflow the candidate list, and the
+// queries of packet waves past their leaf budget): an eighth of the queries, at least 64 Ki.  Past
+// that a near-tie keeps its FP32 result and sets bit 4 of the kernel error flag, and a heavy query
+// stays in its packet wave.
 size_t fixup_entries(size_t nq) { return std::min(nq, std::max<size_t>(1u << 16, nq / 8)); }
+
+}  // namespace
+
+namespace {
+std::atomic<int> &heavy_setting() {
+    static std::atomic<int> v{[] {
+        const char *e = std::getenv("SSDF_HEAVY_LEAVES");
+        if (!e || !*e) return kHeavyLeaves;
+        const long x = std::strtol(e, nullptr, 10);
+        return (int)std::max(0L, std::min(x, 1L << 30));
+    }()};
+    return v;
+}
+}  // namespace
+
+int ssdf_amd::heavy_leaves() { return heavy_setting().load(std::memory_order_relaxed); }
+
+namespace {
 
 int prepare_args(ssdf_mesh *h, QueryArgs &a, size_t nq, unsigned flags, hipStream_t stream) {
     if (h->queried && h->last_stream != stream && h->ws.list) {
@@ -149,12 +170,14 @@ int prepare_args(ssdf_mesh *h, QueryArgs &a, size_t nq, unsigned flags, hipStrea
     }
     h->queried = true;
     h->last_stream = stream;
-    if (int rc = ws_reserve(h->ws, (flags & QF_FIXUP) ? 4 * fixup_entries(nq) : 4, stream)) return rc;
+    if (int rc = ws_reserve(h->ws, 4 * fixup_entries(nq), stream)) return rc;
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, h->ws.status);
     a.fix_count = h->ws.status;
     a.err_flag = (int *)(h->ws.status + 1);
     a.fix_list = h->ws.list;
     a.fix_cap = (unsigned)std::min<size_t>(h->ws.cap / 4, 0xffffffffu);
+    a.heavy_iters = heavy_leaves();
+    a.heavy_cap = a.fix_cap / 2;
     const float L = mesh_scale(h->m);
     a.tau = kTau;
     a.band_abs = kBandRel * L;
@@ -854,6 +877,11 @@ int ssdf_event_elapsed_ms(void *start, void *stop, float *ms) {
     SSDF_HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
     SSDF_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
     return SSDF_OK;
+}
+
+int ssdf_set_heavy_leaves(int leaves) {
+    if (leaves < 0) return heavy_setting().load();
+    return heavy_setting().exchange(leaves);
 }
 
 const char *ssdf_last_error(void) { return get_error(); }

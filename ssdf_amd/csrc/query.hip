@@ -37,8 +37,11 @@ __device__ __forceinline__ float prev_f(float x) {
     return x;
 }
 
+// (sqrt(d2) tau1 + babs)^2: the inclusive pruning bound around a squared distance.  The hardware
+// square root (1 ulp, denormals may flush to 0) is enough: babs, 32 ulps of the coordinate scale,
+// dwarfs its error.  +inf stays +inf.
 __device__ __forceinline__ float band_sq(float d2, float tau1, float babs) {
-    const float r = sqrtf(d2) * tau1 + babs;
+    const float r = __builtin_amdgcn_sqrtf(d2) * tau1 + babs;
     return r * r;
 }
 
@@ -88,6 +91,81 @@ __device__ __forceinline__ T sign_dot(const T qx, const T qy, const T qz, const 
 }
 
 
+// Result of one FP32 query: leaf position k_final (< 0: nothing below the caller's bound), its FP32
+// squared distance bestd, or its FP64 one d2_64 when have64; writes the output of mode OUT (SDF
+// signs by the pseudonormal in FP32, re-evaluated in FP64 near the surface or when the FP32 dot
+// product is too small to trust, so the sign is the FP64 one).  With have64 the output equals
+// exact_write's for the same triangle.
+template <int OUT>
+__device__ __forceinline__ void packet_write(const QueryArgs &a, const long long qid, const float qx, const float qy,
+                                             const float qz, const float out_init, const float init2,
+                                             const float babs, const int k_final, const float bestd,
+                                             const bool have64, const double d2_64) {
+    const TriP *__restrict__ tris = a.tris;
+    if (OUT == OUT_SQEDF) {
+        if (k_final >= 0) {
+            ((float *)a.out_d)[qid] = fminf(init2, have64 ? (float)d2_64 : bestd);
+            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
+        }
+        return;
+    }
+    const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(bestd);
+    if (OUT == OUT_EDF) {
+        if (k_final >= 0) {
+            ((float *)a.out_d)[qid] = fminf(out_init, dist);
+            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
+        }
+        return;
+    }
+    if (k_final < 0) {
+        if (OUT == OUT_SDF) {
+            ((float *)a.out_d)[qid] = SSDF_INF;
+            if (a.out_tri) a.out_tri[qid] = -1;
+        }
+        return;
+    }
+    const TriF tr = tris[k_final].f;
+    if (OUT == OUT_CLOSEST) {
+        if (out_init > dist) {
+            float cx, cy, cz;
+            if (have64) {
+                double ex, ey, ez;
+                pt_tri_cp<double>((double)qx, (double)qy, (double)qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
+                                  tr.b[2], tr.c[0], tr.c[1], tr.c[2], ex, ey, ez);
+                cx = (float)ex, cy = (float)ey, cz = (float)ez;
+            } else {
+                pt_tri_cp<float>(qx, qy, qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2], tr.c[0], tr.c[1],
+                                 tr.c[2], cx, cy, cz);
+            }
+            ((float *)a.out_d)[qid] = dist;
+            ((float *)a.out_cx)[qid] = cx;
+            ((float *)a.out_cy)[qid] = cy;
+            ((float *)a.out_cz)[qid] = cz;
+            if (a.out_tri) a.out_tri[qid] = tr.orig;
+        }
+        return;
+    }
+    bool fp64 = (a.flags & QF_SIGN_FP64) || dist < a.sign_eps;
+    bool neg = false;
+    if (!fp64) {
+        float nlen32;
+        const float dot32 = sign_dot<float>(qx, qy, qz, tr.a, tr.b, tr.c, tr.orig, a.s0, a.s1, a.s2, a.face_n,
+                                            a.edge_n, a.vertex_n, nlen32);
+        neg = dot32 < 0.0f;
+        fp64 = fabsf(dot32) <= (1e-3f * dist + 4.0f * babs) * nlen32;
+    }
+    if (fp64) {
+        const double A[3] = {tr.a[0], tr.a[1], tr.a[2]}, B[3] = {tr.b[0], tr.b[1], tr.b[2]},
+                     C[3] = {tr.c[0], tr.c[1], tr.c[2]};
+        double nl64;
+        const double dot64 = sign_dot<double>((double)qx, (double)qy, (double)qz, A, B, C, tr.orig, a.s0, a.s1,
+                                              a.s2, a.face_n, a.edge_n, a.vertex_n, nl64);
+        neg = dot64 < 0.0;
+    }
+    ((float *)a.out_d)[qid] = neg ? -dist : dist;
+    if (a.out_tri) a.out_tri[qid] = tr.orig;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).
 //
@@ -112,13 +190,15 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     float qx = 0.f, qy = 0.f, qz = 0.f;
     const bool valid = lane_query<QSRC, float>(a, wave_id, lane, qid, qx, qy, qz);
     if (__ballot(valid) == 0) return;
+    bool active = valid;  // false once handed to the cooperative kernel
 
     // FP32 error bound of a distance: 2^-18 (32 ulps of 1) times the largest coordinate involved
     const float tau1 = 1.0f + a.tau;
     const float babs = fmaxf(a.band_abs, 0x1p-18f * fmaxf(fabsf(qx), fmaxf(fabsf(qy), fabsf(qz))));
 
-    // The caller's current distance bounds the search strictly (EDF / CLOSEST).  Plain mode keeps
-    // `best` and prunes with box < best; FIX keeps an inclusive bound thr >= best (its band).
+    // The caller's current distance bounds the search strictly (EDF / CLOSEST / SQEDF).  Plain mode
+    // keeps `best` and prunes with box < best (the first triangle found wins an exact FP32 tie);
+    // FIX keeps an inclusive bound thr >= best, its near-tie band.
     float out_init = SSDF_INF;
     float init2 = SSDF_INF;
     if ((OUT == OUT_EDF || OUT == OUT_CLOSEST) && valid) {
@@ -142,6 +222,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     int iters = 0;
     const int max_iters = (int)a.max_iters;
     int node = uniform_i(a.meta->root);
+    // The leaf budget shares the loop's one iteration test (lim): the hand-over below stays out of
+    // the loop body, where it cost the loop 5-10% even when never taken.
+    int lim = a.heavy_iters > 0 ? min(max_iters, a.heavy_iters) : max_iters;
+    for (;;) {
     for (;;) {
         // descend through internal nodes, nearest child first by majority vote; the decision and the
         // push are uniform selects, so a visit is straight-line code
@@ -211,11 +295,50 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             }
         }
         if (node == kNone || overflow) break;
-        if (++iters > max_iters) break;
+        if (++iters > lim) break;
+    }
+    if (node == kNone || overflow || iters > max_iters) break;
+    lim = max_iters;
+    {
+        // Past the leaf budget the wave's queries disagree too much for one shared walk: each
+        // still active query goes to the queue with the bound found so far (FIX: the FP32 band of
+        // its best; plain: its best) and is searched by a wave of its own.  Queries that find the
+        // hand-over share of the queue full stay here, and the walk goes on.
+        const uint64_t am = __ballot(active);
+        unsigned base = 0, got = 0;
+        if (lane == 0) {
+            // reserve up to the hand-over share of the queue, never past it
+            const unsigned want = (unsigned)__popcll(am);
+            unsigned cur = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (cur < a.heavy_cap) {
+                const unsigned take = min(want, a.heavy_cap - cur);
+                const unsigned prev = atomicCAS(a.fix_count, cur, cur + take);
+                if (prev == cur) {
+                    base = cur, got = take;
+                    break;
+                }
+                cur = prev;
+            }
+        }
+        base = (unsigned)__shfl((int)base, 0);
+        got = (unsigned)__shfl((int)got, 0);
+        const unsigned rank = (unsigned)__popcll(am & ((1ull << lane) - 1ull));
+        if (active && rank < got) {
+            const unsigned slot = base + rank;
+            a.fix_list[4 * slot] = (unsigned)qid;
+            a.fix_list[4 * slot + 1] = __float_as_uint(FIX ? band_sq(bestd, tau1, babs) : best);
+            a.fix_list[4 * slot + 2] = a.fix_tag;
+            a.fix_list[4 * slot + 3] = 0u;
+            active = false;
+            best = -SSDF_INF;
+            thr = -SSDF_INF;
+        }
+        if (__ballot(active) == 0) return;
+    }
     }
     if (lane == 0 && (overflow || iters > max_iters)) atomicOr(a.err_flag, overflow ? 1 : 2);
     if (!FIX) bestd = best;
-    if (!valid) return;
+    if (!active) return;
 
     // ---- resolve the closest triangle ----
     int k_final = bk;      // leaf position
@@ -241,6 +364,13 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             } else {
                 exact_needed = true;  // FP64 rejects every candidate (degenerate triangles)
             }
+        } else if (!exact_needed) {
+            // one candidate: its FP64 distance, so that the output is a function of the query and
+            // the canonical triangle alone, whichever path found it
+            const TriF tr = tris[bk].f;
+            d2_64 = pt_tri_d2<double>((double)qx, (double)qy, (double)qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0],
+                                      tr.b[1], tr.b[2], tr.c[0], tr.c[1], tr.c[2]);
+            have64 = true;
         }
         if (exact_needed) {
             // the exact kernel recomputes this query from the untouched output (its seed), searching
@@ -258,65 +388,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         }
     }
 
-    if (OUT == OUT_SQEDF) {
-        if (k_final >= 0) {
-            ((float *)a.out_d)[qid] = fminf(init2, have64 ? (float)d2_64 : bestd);
-            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
-        }
-        return;
-    }
-    const float dist = have64 ? (float)sqrt(d2_64) : sqrtf(bestd);
-    if (OUT == OUT_EDF) {
-        if (k_final >= 0) {
-            ((float *)a.out_d)[qid] = fminf(out_init, dist);
-            if (a.out_tri) a.out_tri[qid] = tris[k_final].f.orig;
-        }
-        return;
-    }
-    if (k_final < 0) {
-        if (OUT == OUT_SDF) {
-            ((float *)a.out_d)[qid] = SSDF_INF;
-            if (a.out_tri) a.out_tri[qid] = -1;
-        }
-        return;
-    }
-    const TriF tr = tris[k_final].f;
-    if (OUT == OUT_CLOSEST) {
-        if (out_init > dist) {
-            float cx, cy, cz;
-            if (have64) {
-                double ex, ey, ez;
-                pt_tri_cp<double>((double)qx, (double)qy, (double)qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1],
-                                  tr.b[2], tr.c[0], tr.c[1], tr.c[2], ex, ey, ez);
-                cx = (float)ex, cy = (float)ey, cz = (float)ez;
-            } else {
-                pt_tri_cp<float>(qx, qy, qz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2], tr.c[0], tr.c[1],
-                                 tr.c[2], cx, cy, cz);
-            }
-            ((float *)a.out_d)[qid] = dist;
-            ((float *)a.out_cx)[qid] = cx;
-            ((float *)a.out_cy)[qid] = cy;
-            ((float *)a.out_cz)[qid] = cz;
-            if (a.out_tri) a.out_tri[qid] = tr.orig;
-        }
-        return;
-    }
-    // OUT_SDF: sign from the pseudonormal at the closest point; FP64 near the surface or when the
-    // FP32 dot product is too small to trust
-    float nlen32;
-    const float dot32 = sign_dot<float>(qx, qy, qz, tr.a, tr.b, tr.c, tr.orig, a.s0, a.s1, a.s2, a.face_n, a.edge_n,
-                                        a.vertex_n, nlen32);
-    bool neg = dot32 < 0.0f;
-    if ((a.flags & QF_SIGN_FP64) || dist < a.sign_eps || fabsf(dot32) <= (1e-3f * dist + 4.0f * babs) * nlen32) {
-        const double A[3] = {tr.a[0], tr.a[1], tr.a[2]}, B[3] = {tr.b[0], tr.b[1], tr.b[2]},
-                     C[3] = {tr.c[0], tr.c[1], tr.c[2]};
-        double nl64;
-        const double dot64 = sign_dot<double>((double)qx, (double)qy, (double)qz, A, B, C, tr.orig, a.s0, a.s1,
-                                              a.s2, a.face_n, a.edge_n, a.vertex_n, nl64);
-        neg = dot64 < 0.0;
-    }
-    ((float *)a.out_d)[qid] = neg ? -dist : dist;
-    if (a.out_tri) a.out_tri[qid] = tr.orig;
+    packet_write<OUT>(a, qid, qx, qy, qz, out_init, init2, babs, k_final, bestd, have64, d2_64);
 }
 
 template <int STACK, int QSRC, int OUT, bool FIX>
@@ -496,19 +568,32 @@ __device__ void exact_write(const QueryArgs &a, const long long qid, const doubl
 }
 
 // ------------------------------------------------------------------------------------------------
-// Fixup queue, one wave per queued query: the exact FP64 search of exact_one, breadth-first.  The
-// wave expands a frontier of nodes 64 at a time (each lane tests one node's two child boxes in
-// FP64 against the bound and the survivors are compacted into the next frontier by ballots; a
-// leaf is evaluated in FP64 by the lane that holds it), and after every level the bound tightens
-// to the wave's best.  A frontier wider than kFrontier falls back to exact_one.  Same bound, same
-// tie-break (d2, then original index), so the same result as exact_one.
+// Per-query search, one wave per queued query.
+//
+// The queue holds FIX queries whose band has more triangles than the candidate list (word 1: the
+// FP32 band) and the queries of packet waves that ran past their leaf budget (word 1: FIX, the
+// band of the best found so far; plain, the best so far).  The wave walks the tree deepest level
+// first: a step takes up to 64 pending nodes of the deepest non-empty level, one per lane; the
+// lane tests both child boxes against the bound, evaluates a passing leaf at once and appends a
+// passing internal child to the next level.  Children enter a level only while every deeper level
+// is empty, so a level never holds more than 128 entries.  After a step in which some lane
+// improved, the bound tightens to the wave's best.  Waves take entries one at a time from a
+// counter, since their costs differ by orders of magnitude.
+//   FIX (FP64): the search of exact_one: inclusive bound (sqrt(band2) + delta)^2, FP64 distances,
+//     tie-break (d2, original index), so the canonical result.
+//   plain (FP32): inclusive bound (the triangle that gave it is found again), the packet
+//     traversal's FP32 distances, tie-break (d2, original index), the packet's epilogue.  The
+//     packet walk keeps the first triangle it meets among exact FP32 ties and neither walk is
+//     rounding-safe, so the two may differ on ties and by an ulp, as the plain contract allows.
 // ------------------------------------------------------------------------------------------------
-constexpr int kFrontier = 1024;
+constexpr int kCoopBlocks = 1024;  // up to 4 waves each, taking queue entries until none is left
 
-__device__ __forceinline__ void wave_min3(double &d, int &t, int &k) {
+
+template <typename T>
+__device__ __forceinline__ void wave_min3(T &d, int &t, int &k) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        const double od = __shfl_xor(d, off);
+        const T od = __shfl_xor(d, off);
         const int ot = __shfl_xor(t, off), ok = __shfl_xor(k, off);
         if (od < d || (od == d && ot < t)) d = od, t = ot, k = ok;
     }
@@ -519,90 +604,197 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <typename G, int QSRC, int OUT>
-__device__ bool fix_wave_one(const QueryArgs &a, const long long qid, const double band2, int *cur, int *nxt) {
+// box(mn, mx, bound): does a child box pass; leaf(k): evaluates leaf position k into the lane's
+// best, true if it improved; tighten(bound, wave best).  lev: an LDS stack of coop_stack(nlev)
+// ints holding the pending levels one after the other, the deepest on top: only the top level can
+// hold 128 entries, every other one keeps at most the 64 its first step left.  False when the walk
+// needs a level past nlev - 1 (not a tree of that depth).
+__host__ __device__ constexpr int coop_stack(int nlev) { return 64 * (nlev + 1); }
+
+// Waves per block of the per-query kernels: 4 when their stacks fit 64 KiB of LDS, else fewer.
+static int coop_waves(int nlev) {
+    const int per = (int)sizeof(int) * coop_stack(nlev);
+    return std::max(1, std::min(4, 65536 / per));
+}
+
+template <typename T, typename Box, typename Leaf, typename Tighten>
+__device__ __forceinline__ bool coop_walk(const Node *__restrict__ nodes, const int root, int *__restrict__ lev,
+                                          const int nlev, T &bound, T &best, int &bt, int &bk, Box box, Leaf leaf,
+                                          Tighten tighten) {
     const int lane = threadIdx.x & 63;
-    double qx, qy, qz;
-    query_coords<G, QSRC>(a, qid, qx, qy, qz);
-    double out_init = __builtin_inf();
-    if (OUT == OUT_EDF || OUT == OUT_CLOSEST || OUT == OUT_SQEDF) out_init = (double)load_as<G>(a.out_d, qid);
-    const double init2 = OUT == OUT_SQEDF ? out_init : out_init * out_init;  // SQEDF seeds are squared
-    const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
-    double bound;
-    {
-        const double b0 = sqrt(fmin(init2, band2)) + delta;
-        bound = b0 * b0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (root < 0) {  // one triangle: every lane evaluates it
+        leaf(~root);
+        return true;
     }
-    double best = __builtin_inf();
-    int bt = INT_MAX, bk = -1;
-    const int root = uniform_i(a.meta->root);
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    int ncur = 1;
-    for (int level = 0; ncur > 0; level++) {
-        if (level > 4096) return false;  // not a tree
-        int nnext = 0;
-        for (int base = 0; base < ncur; base += 64) {
-            const int idx = base + lane;
-            bool p0 = false, p1 = false;
-            int c0 = 0, c1 = 0;
-            if (idx < ncur) {
-                const int node = level == 0 ? root : cur[idx];
-                if (node < 0) {
-                    const TriF tr = a.tris[~node].f;
-                    double A[3], B[3], C[3];
-                    tri_verts64<G>(a, tr, A, B, C);
-                    const double d =
-                        pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
-                    if (d < init2 && (d < best || (d == best && tr.orig < bt))) best = d, bt = tr.orig, bk = ~node;
-                } else {
-                    const Node nd = a.nodes[node];
-                    const double dl = pt_box_d2<double>(qx, qy, qz, nd.lmin[0], nd.lmin[1], nd.lmin[2], nd.lmax[0],
-                                                        nd.lmax[1], nd.lmax[2]);
-                    const double dr = pt_box_d2<double>(qx, qy, qz, nd.rmin[0], nd.rmin[1], nd.rmin[2], nd.rmax[0],
-                                                        nd.rmax[1], nd.rmax[2]);
-                    p0 = dl <= bound, p1 = dr <= bound;
-                    c0 = nd.left, c1 = nd.right;
-                }
-            }
-            const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
-            const int pos0 = nnext + __popcll(m0 & below);
-            const int pos1 = nnext + __popcll(m0) + __popcll(m1 & below);
-            if (p0 && pos0 < kFrontier) nxt[pos0] = c0;
-            if (p1 && pos1 < kFrontier) nxt[pos1] = c1;
-            nnext += __popcll(m0) + __popcll(m1);
+    // pending entries of level L: lane L of cnt_lo (L < 64) or lane L - 64 of cnt_hi
+    int cnt_lo = lane == 0 ? 1 : 0, cnt_hi = 0;
+    if (lane == 0) lev[0] = root;
+    wave_lds_sync();
+    int top = 0, sp = 1;  // deepest pending level, stack size (wave-uniform)
+    while (top >= 0) {
+        const int c = top < 64 ? __builtin_amdgcn_readlane(cnt_lo, top) : __builtin_amdgcn_readlane(cnt_hi, top - 64);
+        const int take = min(c, 64);
+        if (top < 64)
+            cnt_lo = lane == top ? c - take : cnt_lo;
+        else
+            cnt_hi = lane == top - 64 ? c - take : cnt_hi;
+        sp -= take;
+        bool p0 = false, p1 = false, imp = false;
+        int c0 = 0, c1 = 0;
+        if (lane < take) {
+            const Node nd = nodes[lev[sp + lane]];
+            p0 = box(nd.lmin, nd.lmax, bound);
+            p1 = box(nd.rmin, nd.rmax, bound);
+            c0 = nd.left, c1 = nd.right;
+            if (p0 && c0 < 0) imp |= leaf(~c0), p0 = false;
+            if (p1 && c1 < 0) imp |= leaf(~c1), p1 = false;
         }
-        double wb = best;
-        int wt = bt, wk = bk;
-        wave_min3(wb, wt, wk);
-        if (wb < __builtin_inf()) {
-            const double r = sqrt(wb) + delta;
-            bound = fmin(bound, r * r);
+        const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+        const int n = __popcll(m0) + __popcll(m1);
+        if (n > 0) {
+            if (top + 1 >= nlev || sp + n > coop_stack(nlev)) return false;
+            int *dst = lev + sp;
+            if (p0) dst[__popcll(m0 & below)] = c0;
+            if (p1) dst[__popcll(m0) + __popcll(m1 & below)] = c1;
+            sp += n;
+            top++;
+            if (top < 64)
+                cnt_lo = lane == top ? n : cnt_lo;
+            else
+                cnt_hi = lane == top - 64 ? n : cnt_hi;
+        } else {
+            const uint64_t h0 = __ballot(cnt_lo > 0), h1 = __ballot(cnt_hi > 0);
+            top = h1 ? 127 - __clzll(h1) : (h0 ? 63 - __clzll(h0) : -1);
         }
-        if (nnext > kFrontier) return false;
+        if (__ballot(imp) != 0) {
+            T wb = best;
+            int wt = bt, wk = bk;
+            wave_min3(wb, wt, wk);
+            tighten(bound, wb);
+        }
         wave_lds_sync();
-        int *t = cur;
-        cur = nxt, nxt = t;
-        ncur = nnext;
     }
-    wave_min3(best, bt, bk);
-    if (lane == 0) exact_write<G, OUT>(a, qid, qx, qy, qz, out_init, best, bt, bk);
     return true;
 }
 
-constexpr int kFixWaves = 4;
+// FIX entry: the exact FP64 search below band2, result written as exact_one writes it.
+template <int QSRC, int OUT>
+__device__ bool coop_exact(const QueryArgs &a, const long long qid, const double band2, int *lev, const int nlev) {
+    double qx, qy, qz;
+    query_coords<float, QSRC>(a, qid, qx, qy, qz);
+    double out_init = __builtin_inf();
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST || OUT == OUT_SQEDF) out_init = (double)load_as<float>(a.out_d, qid);
+    const double init2 = OUT == OUT_SQEDF ? out_init : out_init * out_init;  // SQEDF seeds are squared
+    const double delta = fmax(a.delta64, 1e-13 * fmax(fabs(qx), fmax(fabs(qy), fabs(qz))));
+    const double b0 = sqrt(fmin(init2, band2)) + delta;
+    double bound = b0 * b0;
+    double best = __builtin_inf();
+    int bt = INT_MAX, bk = -1;
+    const bool ok = coop_walk<double>(
+        a.nodes, uniform_i(a.meta->root), lev, nlev, bound, best, bt, bk,
+        [&](const float *mn, const float *mx, const double bnd) {
+            return pt_box_d2<double>(qx, qy, qz, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]) <= bnd;
+        },
+        [&](const int k) {
+            const TriF tr = a.tris[k].f;
+            double A[3], B[3], C[3];
+            tri_verts64<float>(a, tr, A, B, C);
+            const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
+            if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
+                best = d, bt = tr.orig, bk = k;
+                return true;
+            }
+            return false;
+        },
+        [&](double &bnd, const double wb) {
+            if (wb < __builtin_inf()) {
+                const double r = sqrt(wb) + delta;
+                bnd = fmin(bnd, r * r);
+            }
+        });
+    if (!ok) return false;
+    wave_min3(best, bt, bk);
+    if ((threadIdx.x & 63) == 0) exact_write<float, OUT>(a, qid, qx, qy, qz, out_init, best, bt, bk);
+    return true;
+}
+
+// Plain entry: the FP32 minimum at or below bound0 (the packet wave's best so far), written by the
+// packet epilogue.
+template <int QSRC, int OUT>
+__device__ bool coop_plain(const QueryArgs &a, const long long qid, const float bound0, int *lev, const int nlev) {
+    double dx, dy, dz;
+    query_coords<float, QSRC>(a, qid, dx, dy, dz);  // FP32 coordinates, widened exactly
+    const float qx = (float)dx, qy = (float)dy, qz = (float)dz;
+    float out_init = SSDF_INF, init2 = SSDF_INF;
+    if (OUT == OUT_EDF || OUT == OUT_CLOSEST) {
+        out_init = ((const float *)a.out_d)[qid];
+        init2 = out_init * out_init;
+    }
+    if (OUT == OUT_SQEDF) init2 = ((const float *)a.out_d)[qid];
+    const float babs = fmaxf(a.band_abs, 0x1p-18f * fmaxf(fabsf(qx), fmaxf(fabsf(qy), fabsf(qz))));
+    float bound = bound0, best = SSDF_INF;
+    int bt = INT_MAX, bk = -1;
+    const TriP *__restrict__ tris = a.tris;
+    const bool ok = coop_walk<float>(
+        a.nodes, uniform_i(a.meta->root), lev, nlev, bound, best, bt, bk,
+        [&](const float *mn, const float *mx, const float bnd) { return box_d2_fast(qx, qy, qz, mn, mx) <= bnd; },
+        [&](const int k) {
+            const TriP &tr = tris[k];
+            const float d2 = tri_d2_fast(qx, qy, qz, tr);
+            const int orig = tr.f.orig;
+            if (d2 < init2 && (d2 < best || (d2 == best && orig < bt))) {
+                best = d2, bt = orig, bk = k;
+                return true;
+            }
+            return false;
+        },
+        [&](float &bnd, const float wb) { bnd = fminf(bnd, wb); });
+    if (!ok) return false;
+    wave_min3(best, bt, bk);
+    if ((threadIdx.x & 63) == 0) packet_write<OUT>(a, qid, qx, qy, qz, out_init, init2, babs, bk, best, false, 0.0);
+    return true;
+}
+
+template <int QSRC, int OUT, bool FIX>
+__device__ __forceinline__ void coop_item(const QueryArgs &a, const unsigned *e, int *lev, const int nlev) {
+    const long long qid = (long long)e[0];
+    const float b = __uint_as_float(e[1]);
+    const bool ok = FIX ? coop_exact<QSRC, OUT>(a, qid, (double)b, lev, nlev)
+                        : coop_plain<QSRC, OUT>(a, qid, b, lev, nlev);
+    if (!ok && (threadIdx.x & 63) == 0) atomicOr(a.err_flag, 1);
+    wave_lds_sync();  // the next item reuses the levels
+}
+
+// Next queue entry for the calling wave (the entries differ in cost by orders of magnitude, so
+// they are handed out one at a time); n_items or more when the queue is exhausted.  A plain read
+// first, so that waves arriving after the end (or at an empty queue: most calls) take no atomic.
+__device__ __forceinline__ unsigned coop_next(const QueryArgs &a, const unsigned n_items) {
+    unsigned item = 0;
+    if ((threadIdx.x & 63) == 0) {
+        item = __hip_atomic_load(a.fix_count + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (item < n_items) item = atomicAdd(a.fix_count + 2, 1u);
+    }
+    return (unsigned)__shfl((int)item, 0);
+}
+
+template <int QSRC, int OUT, bool FIX>
+__global__ __launch_bounds__(256) void k_coop(QueryArgs a, int nlev) {
+    extern __shared__ int lds_levels[];
+    int *lev = lds_levels + (threadIdx.x >> 6) * coop_stack(nlev);
+    const unsigned n_items = min(*a.fix_count, a.fix_cap);
+    for (unsigned item = coop_next(a, n_items); item < n_items; item = coop_next(a, n_items))
+        coop_item<QSRC, OUT, FIX>(a, a.fix_list + 4 * (size_t)item, lev, nlev);
+}
 
 template <int QSRC, int OUT>
-__global__ __launch_bounds__(64 * kFixWaves) void k_fix_wave(QueryArgs a) {
-    __shared__ int fr[kFixWaves][2][kFrontier];
-    const int w = threadIdx.x >> 6;
-    const unsigned n_items = min(*a.fix_count, a.fix_cap);
-    for (unsigned item = blockIdx.x * kFixWaves + w; item < n_items; item += gridDim.x * kFixWaves) {
-        const unsigned *e = a.fix_list + 4 * (size_t)item;
-        const long long qid = (long long)e[0];
-        const double band2 = (double)__uint_as_float(e[1]);
-        if (!fix_wave_one<float, QSRC, OUT>(a, qid, band2, fr[w][0], fr[w][1]))
-            if ((threadIdx.x & 63) == 0) exact_one<float, QSRC, OUT>(a, qid, band2);
-    }
+static void launch_coop(const QueryArgs &a, bool fix, int nlev, hipStream_t s) {
+    const int w = coop_waves(nlev);
+    const size_t lds = sizeof(int) * (size_t)coop_stack(nlev) * w;
+    if (fix)
+        hipLaunchKernelGGL((k_coop<QSRC, OUT, true>), dim3(kCoopBlocks), dim3(64 * w), lds, s, a, nlev);
+    else
+        hipLaunchKernelGGL((k_coop<QSRC, OUT, false>), dim3(kCoopBlocks), dim3(64 * w), lds, s, a, nlev);
 }
 
 // QUEUE: the items are the fixup queue's entries (query id, FP32 band); else every query.
@@ -691,12 +883,12 @@ __global__ __launch_bounds__(256) void k_batch_packet(const Node *__restrict__ n
     }
 }
 
-template <int QSRC, int OUT>
-__global__ __launch_bounds__(64 * kFixWaves) void k_batch_exact(const BatchMesh *__restrict__ meshes, BatchArgs b) {
-    __shared__ int fr[kFixWaves][2][kFrontier];
-    const int w = threadIdx.x >> 6;
+template <int QSRC, int OUT, bool FIX>
+__global__ __launch_bounds__(256) void k_batch_coop(const BatchMesh *__restrict__ meshes, BatchArgs b, int nlev) {
+    extern __shared__ int lds_levels[];
+    int *lev = lds_levels + (threadIdx.x >> 6) * coop_stack(nlev);
     const unsigned n_items = min(*b.base.fix_count, b.base.fix_cap);
-    for (unsigned item = blockIdx.x * kFixWaves + w; item < n_items; item += gridDim.x * kFixWaves) {
+    for (unsigned item = coop_next(b.base, n_items); item < n_items; item = coop_next(b.base, n_items)) {
         const unsigned *e = b.base.fix_list + 4 * (size_t)item;
         const int mi = (int)e[2];
         // descriptors are ordered by unit, not by mesh: find the mesh's entry
@@ -704,16 +896,13 @@ __global__ __launch_bounds__(64 * kFixWaves) void k_batch_exact(const BatchMesh 
         while (j < b.nmeshes - 1 && meshes[j].mesh != mi) j++;
         QueryArgs a;
         batch_args(b, meshes[j], a);
-        const long long qid = (long long)e[0];
-        const double band2 = (double)__uint_as_float(e[1]);
-        if (!fix_wave_one<float, QSRC, OUT>(a, qid, band2, fr[w][0], fr[w][1]))
-            if ((threadIdx.x & 63) == 0) exact_one<float, QSRC, OUT>(a, qid, band2);
+        coop_item<QSRC, OUT, FIX>(a, e, lev, nlev);
     }
 }
 
 __global__ void k_batch_reset(unsigned *queue, unsigned *status) {
     if (threadIdx.x < 8) queue[64 * threadIdx.x] = 0u;
-    if (threadIdx.x < 2) status[threadIdx.x] = 0u;
+    if (threadIdx.x < 4) status[threadIdx.x] = 0u;
 }
 
 template <int STACK, int QSRC, int OUT>
@@ -725,7 +914,14 @@ static void launch_batch_packet(const BatchArgs &b, const BatchMesh *meshes, boo
     else
         hipLaunchKernelGGL((k_batch_packet<STACK, QSRC, OUT, false>), dim3(nblocks), dim3(256), 0, s, b.nodes,
                            b.tris, meshes, b);
-    if (fix) hipLaunchKernelGGL((k_batch_exact<QSRC, OUT>), dim3(512), dim3(64 * kFixWaves), 0, s, meshes, b);
+    const int w = coop_waves(b.nlev);
+    const size_t lds = sizeof(int) * (size_t)coop_stack(b.nlev) * w;
+    if (fix)
+        hipLaunchKernelGGL((k_batch_coop<QSRC, OUT, true>), dim3(kCoopBlocks), dim3(64 * w), lds, s, meshes, b,
+                           b.nlev);
+    else
+        hipLaunchKernelGGL((k_batch_coop<QSRC, OUT, false>), dim3(kCoopBlocks), dim3(64 * w), lds, s, meshes, b,
+                           b.nlev);
 }
 
 template <int STACK, int QSRC>
@@ -909,25 +1105,25 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
         return SSDF_ERR_DEPTH;
     }
     if (rc) return rc;
-    if (fix) {
+    if (fix || a.heavy_iters > 0) {
         // queued queries: one wave each, grid-stride over the device-side count
-        const dim3 g(512), blk(64 * kFixWaves);
+        const int nlev = std::max(m.hmeta.maxdepth, 1);
         if (qsrc == QSRC_POINTS) {
             if (out == OUT_EDF)
-                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_EDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_POINTS, OUT_EDF>(a, fix, nlev, stream);
             else if (out == OUT_SDF)
-                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_SDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_POINTS, OUT_SDF>(a, fix, nlev, stream);
             else if (out == OUT_SQEDF)
-                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_SQEDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_POINTS, OUT_SQEDF>(a, fix, nlev, stream);
             else
-                hipLaunchKernelGGL((k_fix_wave<QSRC_POINTS, OUT_CLOSEST>), g, blk, 0, stream, a);
+                launch_coop<QSRC_POINTS, OUT_CLOSEST>(a, fix, nlev, stream);
         } else {
             if (out == OUT_EDF)
-                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_EDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_GRID, OUT_EDF>(a, fix, nlev, stream);
             else if (out == OUT_SQEDF)
-                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_SQEDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_GRID, OUT_SQEDF>(a, fix, nlev, stream);
             else
-                hipLaunchKernelGGL((k_fix_wave<QSRC_GRID, OUT_SDF>), g, blk, 0, stream, a);
+                launch_coop<QSRC_GRID, OUT_SDF>(a, fix, nlev, stream);
         }
     }
     SSDF_HIP_TRY(hipGetLastError());

@@ -50,12 +50,14 @@ struct QueryArgs {
     float sign_eps; // FP64 sign below this distance
     double delta64; // FP64 pruning margin of the exact traversal
     // fixup queue: query ids whose candidate list overflowed
-    unsigned *fix_count;
-    unsigned *fix_list;  // 4 words per entry: query id, FP32 band (bits), tag (mesh of a batch), 0
-    unsigned fix_cap;    // entries
+    unsigned *fix_count;  // [0] queue entries written, [2] next entry for the per-query kernel
+    unsigned *fix_list;   // 4 words per entry: query id, FP32 bound (bits), tag (mesh of a batch), 0
+    unsigned fix_cap;     // entries
+    unsigned heavy_cap;   // entries packet waves may hand over (the rest is kept for FIX near-ties)
     unsigned fix_tag;
     int *err_flag;  // bit 1: a traversal stack would overflow; bit 2: iteration cap hit; bit 4: fixup queue full
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
+    int heavy_iters;      // leaves a packet wave walks before handing its queries over (0: never)
     unsigned nblocks;  // launched blocks (XCD remap)
     unsigned flags;    // QF_* of the call
     // winding numbers (gwn.hip)
@@ -73,6 +75,16 @@ constexpr float kBandRel = 0x1p-18f;    // FP32 error bound of a distance, relat
 constexpr float kTau = 1e-5f;           // relative near-tie band on distances
 constexpr double kDelta64Rel = 1e-13;   // FP64 pruning margin relative to the coordinate scale
 constexpr double kSignEpsRel = 1e-6;    // FP64 sign within this fraction of the bbox diagonal
+// Leaves a packet wave walks before its queries go to the per-query kernel.  A wave of grid nodes
+// outside or near a surface needs tens to a few thousand; deep inside a bumpy near-sphere, where
+// the nearest regions differ from lane to lane, the shared walk grows to 10^5 leaves and more and
+// one wave would run for hundreds of milliseconds.  Measured best of 1 Ki - 16 Ki on grids and
+// point clouds around 1.4-2 M-triangle near-spheres and on 64 near-spheres of 1 k-500 k
+// triangles; smaller budgets fill the hand-over share of the queue with moderately incoherent
+// waves before the heavy ones arrive.
+constexpr int kHeavyLeaves = 8192;
+// The current budget (ssdf_set_heavy_leaves; initially kHeavyLeaves or SSDF_HEAVY_LEAVES).
+int heavy_leaves();
 
 // Coordinate scale max(1, |vertex bounds|) and bounding-box diagonal of a built mesh.
 inline float meta_scale(const MeshMeta &hm) {
@@ -123,9 +135,10 @@ struct BatchArgs {
     unsigned total_units;
     unsigned *queue;  // 8 counters (one per XCD range), 64 words apart
     unsigned chunk;   // units per grab
+    int nlev;         // LDS levels of the per-query kernel: the deepest tree of the batch
 };
 
-// Launches the reset, the persistent packet kernel and (FIX) the fixup-queue kernel of a batch.
+// Launches the reset, the persistent packet kernel and the queue kernel of a batch.
 // meshes: device descriptor array of b.nmeshes entries; stack: 64 or 128.
 int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out, bool fix, int stack,
                  unsigned nblocks, hipStream_t stream);
