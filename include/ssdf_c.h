@@ -177,6 +177,8 @@ typedef struct {
 int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *s1, const int *s2,
                      ptrdiff_t nspoints, const void *sx, const void *sy, const void *sz, int device_ptrs,
                      int with_normals, void *stream, ssdf_mesh_t *out);
+/* Waits for the device to finish and frees the handle; it does not use the streams the handle was
+ * created or queried on, so it is safe after the caller has destroyed them. */
 int ssdf_mesh_destroy(ssdf_mesh_t mesh);
 /* Rebuilds the LBVH (and, with_normals != 0, the pseudonormals) from the handle's current arrays,
  * reusing its device memory: no allocation after the first build.  For meshes whose vertices move
@@ -230,6 +232,8 @@ typedef struct ssdf_batch *ssdf_batch_t;
 int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t *vert_offsets, const int *s0,
                       const int *s1, const int *s2, const float *sx, const float *sy, const float *sz, int device_ptrs,
                       int with_normals, void *stream, ssdf_batch_t *out);
+/* The creation stream orders the batch's own allocations and must outlive its use; destroy waits
+ * for the device instead and is safe after the caller has destroyed its streams. */
 int ssdf_batch_destroy(ssdf_batch_t batch);
 /* Rebuilds every mesh's LBVH (and normals) from the batch's vertex arena, in place. */
 int ssdf_batch_rebuild(ssdf_batch_t batch, int with_normals, void *stream);

@@ -50,12 +50,14 @@ struct ssdf_batch {
 
 namespace {
 
+// The caller may have destroyed the batch's creation stream already: the device is synchronized
+// and the frees go to the null stream.
 void destroy_batch(ssdf_batch *b) {
     if (!b) return;
-    const hipStream_t st = b->stream;
+    (void)hipDeviceSynchronize();
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
-    (void)hipStreamSynchronize(st);
+    const hipStream_t st = nullptr;
     for (void *p : {(void *)b->s0, (void *)b->s1, (void *)b->s2, (void *)b->sx, (void *)b->sy, (void *)b->sz,
                     (void *)b->nodes, (void *)b->tris, (void *)b->metas, (void *)b->face_n, (void *)b->edge_n,
                     (void *)b->vertex_n, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,

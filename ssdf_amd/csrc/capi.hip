@@ -204,7 +204,7 @@ int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const i
     if (!rc && nsel > 0 && with_normals) rc = build_normals(h->m, stream);
     if (!rc && nsel > 0) rc = sync_meta(h->m, stream);
     if (rc) {
-        free_mesh(h->m);
+        free_mesh(h->m, stream);
         delete h;
         return rc;
     }
@@ -212,20 +212,29 @@ int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const i
     return SSDF_OK;
 }
 
-void destroy_mesh(ssdf_mesh *h) {
+// streams_alive: the creation and last-query streams still exist (the library's own callers), so
+// the frees are ordered behind the handle's work on them.  Otherwise (the public destroy: the
+// caller may have destroyed its streams already) the device is synchronized and the frees go to
+// the null stream.
+void destroy_mesh(ssdf_mesh *h, bool streams_alive) {
     if (!h) return;
-    const hipStream_t st = h->m.stream;
-    if (h->queried && h->last_stream != st) {
-        // frees are ordered on the creation stream: make it wait for the last query's stream
-        hipEvent_t ev;
-        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-            (void)hipEventRecord(ev, h->last_stream);
-            (void)hipStreamWaitEvent(st, ev, 0);
-            (void)hipEventDestroy(ev);
+    hipStream_t st = nullptr;
+    if (streams_alive) {
+        st = h->m.stream;
+        if (h->queried && h->last_stream != st) {
+            // frees are ordered on the creation stream: make it wait for the last query's stream
+            hipEvent_t ev;
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+                (void)hipEventRecord(ev, h->last_stream);
+                (void)hipStreamWaitEvent(st, ev, 0);
+                (void)hipEventDestroy(ev);
+            }
         }
+    } else {
+        (void)hipDeviceSynchronize();
     }
     ws_free(h->ws, st);
-    free_mesh(h->m);
+    free_mesh(h->m, st);
     delete h;
 }
 
@@ -397,7 +406,7 @@ int host_points_call(int mode, ptrdiff_t npoints, const G *x, const G *y, const 
         return rc;
     struct Guard {
         ssdf_mesh *h;
-        ~Guard() { destroy_mesh(h); }
+        ~Guard() { destroy_mesh(h, true); }
     } guard{h};
     DevBuf dx, dy, dz, dd, dt, dcx, dcy, dcz;
     if (int rc = h2d(dx, x, npoints)) return rc;
@@ -443,7 +452,7 @@ int host_grid_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, 
     if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, true, stream, &h)) return rc;
     struct Guard {
         ssdf_mesh *h;
-        ~Guard() { destroy_mesh(h); }
+        ~Guard() { destroy_mesh(h, true); }
     } guard{h};
     const ptrdiff_t nq = nx * ny * nz;
     DevBuf dd, dt;
@@ -477,7 +486,7 @@ int host_sqedf_call(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2,
     if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
     struct Guard {
         ssdf_mesh *h;
-        ~Guard() { destroy_mesh(h); }
+        ~Guard() { destroy_mesh(h, true); }
     } guard{h};
     DevBuf dd;
     if (int rc = h2d(dd, out, nq)) return rc;
@@ -507,7 +516,7 @@ int host_inside_points(ptrdiff_t npoints, const G *x, const G *y, const G *z, pt
     if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
     struct Guard {
         ssdf_mesh *h;
-        ~Guard() { destroy_mesh(h); }
+        ~Guard() { destroy_mesh(h, true); }
     } guard{h};
     DevBuf dx, dy, dz, dout;
     if (int rc = h2d(dx, x, npoints)) return rc;
@@ -541,7 +550,7 @@ int host_inside_grid(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2
     if (int rc = create_mesh(dtype, nsel, s0, s1, s2, nsp, sx, sy, sz, false, false, stream, &h)) return rc;
     struct Guard {
         ssdf_mesh *h;
-        ~Guard() { destroy_mesh(h); }
+        ~Guard() { destroy_mesh(h, true); }
     } guard{h};
     DevBuf dout;
     if (int rc = h2d(dout, (const uint8_t *)nullptr, (ptrdiff_t)nq)) return rc;
@@ -554,6 +563,9 @@ int host_inside_grid(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2
 }  // namespace
 
 namespace ssdf_amd {
+// Destroys a mesh of server_mesh_create, its frees ordered on the server's own (live) streams.
+void server_mesh_destroy(ssdf_mesh_t h) { destroy_mesh(h, true); }
+
 // Resident FP32 mesh with normals from host arrays, for the serving loop (server.hip).
 int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t nsel, const float *sx,
                        const float *sy, const float *sz, ptrdiff_t nsp, hipStream_t stream, ssdf_mesh_t *out,
@@ -751,7 +763,7 @@ int ssdf_mesh_update_vertices(ssdf_mesh_t mesh, const void *sx, const void *sy, 
 }
 
 int ssdf_mesh_destroy(ssdf_mesh_t mesh) {
-    destroy_mesh(mesh);
+    destroy_mesh(mesh, false);
     return SSDF_OK;
 }
 

@@ -647,8 +647,7 @@ int build_normals(DevMesh &m, hipStream_t stream) {
     return SSDF_OK;
 }
 
-void free_mesh(DevMesh &m) {
-    hipStream_t st = m.stream;
+void free_mesh(DevMesh &m, hipStream_t st) {
     if (m.owns_input) {
         dev_free(m.s0, st);
         dev_free(m.s1, st);

@@ -63,7 +63,9 @@ int build_normals(DevMesh &m, hipStream_t stream);
 int build_gwn(DevMesh &m, hipStream_t stream);
 // Copies meta back (synchronises the stream).
 int sync_meta(DevMesh &m, hipStream_t stream);
-void free_mesh(DevMesh &m);
+// Frees the mesh's device arrays, ordered on `st` (a stream that is still alive; null: the null
+// stream, for callers that have synchronized the device first).
+void free_mesh(DevMesh &m, hipStream_t st);
 
 // thread-local last error message of the C ABI
 void set_error(const std::string &msg);

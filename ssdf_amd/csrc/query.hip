@@ -25,7 +25,11 @@
 namespace ssdf_amd {
 
 constexpr int kWaves = 4;
-constexpr int kCand = 24;  // 24 KB of LDS per block: six blocks per CU, as the FP32+fixup VGPR count allows
+// FIX candidate list: 16 KB of LDS per block, so that LDS allows 8 waves per SIMD (the packet
+// kernels ask the compiler for 8: 64 VGPRs).  Measured on the 200 k-triangle torus at 256^3: 16
+// entries and 8 waves take 7.7 ms, 24 entries and 6 waves 8.4 ms; a query with more than 16
+// triangles in its band goes to the per-query kernel.
+constexpr int kCand = 16;
 constexpr int kNone = INT_MIN;
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -328,7 +332,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             a.fix_list[4 * slot] = (unsigned)qid;
             a.fix_list[4 * slot + 1] = __float_as_uint(FIX ? band_sq(bestd, tau1, babs) : best);
             a.fix_list[4 * slot + 2] = a.fix_tag;
-            a.fix_list[4 * slot + 3] = 0u;
+            a.fix_list[4 * slot + 3] = (unsigned)((unsigned long long)qid >> 32);
             active = false;
             best = -SSDF_INF;
             thr = -SSDF_INF;
@@ -380,7 +384,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
                 a.fix_list[4 * pos] = (unsigned)qid;
                 a.fix_list[4 * pos + 1] = __float_as_uint(band_sq(bestd, tau1, babs));
                 a.fix_list[4 * pos + 2] = a.fix_tag;
-                a.fix_list[4 * pos + 3] = 0u;
+                a.fix_list[4 * pos + 3] = (unsigned)((unsigned long long)qid >> 32);
                 return;
             }
             // queue full: keep the FP32 result and report it (error bit 4)
@@ -392,7 +396,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
 }
 
 template <int STACK, int QSRC, int OUT, bool FIX>
-__global__ __launch_bounds__(256) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+// 8 waves per SIMD (64 VGPRs): FIX spills 17 VGPRs and 30 SGPRs (none inside the descent loop; the
+// leaf and pop code reloads some) and still gains 8% on the 200 k-triangle torus at 256^3 over its
+// unconstrained 78 VGPRs and 6 waves.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
     const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
@@ -758,7 +765,7 @@ __device__ bool coop_plain(const QueryArgs &a, const long long qid, const float 
 
 template <int QSRC, int OUT, bool FIX>
 __device__ __forceinline__ void coop_item(const QueryArgs &a, const unsigned *e, int *lev, const int nlev) {
-    const long long qid = (long long)e[0];
+    const long long qid = (long long)(((unsigned long long)e[3] << 32) | e[0]);
     const float b = __uint_as_float(e[1]);
     const bool ok = FIX ? coop_exact<QSRC, OUT>(a, qid, (double)b, lev, nlev)
                         : coop_plain<QSRC, OUT>(a, qid, b, lev, nlev);
@@ -806,7 +813,9 @@ __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
     for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
          item += (long long)gridDim.x * blockDim.x) {
         if (QUEUE)
-            exact_one<G, QSRC, OUT>(a, (long long)a.fix_list[4 * item],
+            exact_one<G, QSRC, OUT>(a,
+                                    (long long)(((unsigned long long)a.fix_list[4 * item + 3] << 32) |
+                                                a.fix_list[4 * item]),
                                     (double)__uint_as_float(a.fix_list[4 * item + 1]));
         else
             exact_one<G, QSRC, OUT>(a, item, __builtin_inf());
@@ -847,7 +856,9 @@ __device__ __forceinline__ void batch_args(const BatchArgs &b, const BatchMesh &
 // XCD the hardware dispatches it to, so each XCD's L2 holds mostly its range's meshes) and moves
 // on to the next ranges when its own is exhausted.
 template <int STACK, int QSRC, int OUT, bool FIX>
-__global__ __launch_bounds__(256) void k_batch_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+// Occupancy, measured on the 64-mesh batch: plain runs best at 8 waves per SIMD (64 VGPRs), FIX
+// unconstrained (93 VGPRs, 5 waves): forced to 8 it spills 42 VGPRs and is 2% slower.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FIX ? 1 : 8, 8))) void k_batch_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       const BatchMesh *__restrict__ meshes, BatchArgs b) {
     __shared__ int cand[FIX ? kCand : 1][256];
     const int lane = threadIdx.x & 63;

@@ -51,7 +51,8 @@ struct QueryArgs {
     double delta64; // FP64 pruning margin of the exact traversal
     // fixup queue: query ids whose candidate list overflowed
     unsigned *fix_count;  // [0] queue entries written, [2] next entry for the per-query kernel
-    unsigned *fix_list;   // 4 words per entry: query id, FP32 bound (bits), tag (mesh of a batch), 0
+    unsigned *fix_list;   // 4 words per entry: query id (low 32 bits), FP32 bound (bits), tag (mesh of
+                          // a batch), query id (high 32 bits)
     unsigned fix_cap;     // entries
     unsigned heavy_cap;   // entries packet waves may hand over (the rest is kept for FIX near-ties)
     unsigned fix_tag;

@@ -31,6 +31,8 @@ namespace ssdf_amd {
 int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t nsel, const float *sx,
                        const float *sy, const float *sz, ptrdiff_t nsp, hipStream_t stream, ssdf_mesh_t *out,
                        size_t *bytes);
+// Destroys a mesh of server_mesh_create, its frees ordered on the server's (live) streams.
+void server_mesh_destroy(ssdf_mesh_t h);
 }  // namespace ssdf_amd
 
 using namespace ssdf_amd;
@@ -121,7 +123,7 @@ bool make_room(ssdf_server *s, size_t need) {
             }
         if (victim < 0) return false;
         MeshEntry &e = s->meshes[victim];
-        ssdf_mesh_destroy(e.handle);
+        server_mesh_destroy(e.handle);
         e.handle = nullptr;
         e.resident = false;
         s->lru.erase(e.lru);
@@ -433,7 +435,7 @@ int ssdf_server_destroy(ssdf_server_t s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     for (auto &kv : s->meshes)
-        if (kv.second.handle) ssdf_mesh_destroy(kv.second.handle);
+        if (kv.second.handle) server_mesh_destroy(kv.second.handle);
     for (auto &kv : s->by_ticket) delete kv.second;
     for (int i = 0; i < kStreams; i++) (void)hipStreamDestroy(s->streams[i]);
     delete s;

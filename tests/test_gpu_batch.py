@@ -165,3 +165,28 @@ def test_batch_autotune_and_schedule(meshes):
     assert info["nselements"] == sum(m.ntris for m in meshes)
     with pytest.raises(S.SsdfError):
         b.set_schedule(99, 1)
+
+
+def test_destroy_after_the_stream_is_gone(meshes):
+    """Destroying a batch or a resident mesh does not touch the streams it was created or queried
+    on: the caller may have destroyed them first (as Python's shutdown can)."""
+    st = S.Stream()
+    b = S.Batch(meshes[:3], stream=st)
+    dims, boxes = _grids(meshes[:3])
+    off, total = S.Batch.grid_layout(dims)
+    out = S.DeviceArray(total, np.float32)
+    b.query_grids(S.MODE_EDF, dims, boxes, off, out, None, flags=S.FLAG_FIXUP, stream=st)
+    rm = S.ResidentMesh.from_mesh(meshes[1], stream=st)
+    o = S.DeviceArray(8 ** 3, np.float32)
+    rm.query_grid(S.MODE_SDF, (8, 8, 8), (-1, -1, -1), (1, 1, 1), o, None, flags=S.FLAG_FIXUP, stream=st)
+    st.synchronize()
+    S._check(S.lib().ssdf_stream_destroy(st.ptr), "ssdf_stream_destroy")
+    st.ptr = None
+    b.close()
+    rm.close()
+    # the device is still usable afterwards
+    rm2 = S.ResidentMesh.from_mesh(meshes[0])
+    o2 = S.DeviceArray(8 ** 3, np.float32)
+    rm2.query_grid(S.MODE_SDF, (8, 8, 8), (-1, -1, -1), (1, 1, 1), o2, None, flags=S.FLAG_FIXUP)
+    assert np.all(np.isfinite(o2.download()))
+    rm2.close()
