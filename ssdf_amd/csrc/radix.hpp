@@ -1,11 +1,15 @@
 // radix.hpp — stable LSD radix sort of (key, uint32 value) pairs on gfx950.
 //
-// One pass per 8-bit digit, three kernels per pass:
-//   hist    : per-tile digit histogram in LDS, written digit-major (d * ntiles + tile);
-//   scan    : one-block exclusive scan of that digit-major table -> global scatter offsets;
+// One pass per 8-bit digit:
+//   hist    : per-tile digit histogram in LDS;
+//   scan    : exclusive scan of the (digit, tile) table -> global scatter offsets;
 //   scatter : each wave ranks its 16 chunks of 64 keys in order — the lanes holding the same digit
 //             are found with 8 ballots (one per digit bit), a lane's rank is the popcount of its
 //             peers below it — so the pass is stable without any LDS sort.
+// Up to kFuseTiles tiles the scan lives inside the other two kernels: hist writes its counts
+// tile-major and adds them into the pass's digit totals (zeroed once per sort), and each scatter
+// block scans the 256 totals and sums the earlier tiles' counts itself (coalesced, L2-resident).
+// Larger sorts take a third, one-block scan kernel over a digit-major table.
 // Tiles are 4096 keys (256 threads x 16).  All launches go to the caller's stream and nothing is
 // read back, so a sort can sit inside a captured graph.
 #pragma once
@@ -18,14 +22,27 @@ constexpr int kRadixBlock = 256;
 constexpr int kRadixItems = 16;
 constexpr int kRadixTile = kRadixBlock * kRadixItems;
 
+// Sorts of at most this many tiles fold the scan into hist and scatter.
+constexpr unsigned kFuseTiles = 192;
+constexpr int kMaxPasses = 8;
+
 inline unsigned radix_tiles(size_t n) { return (unsigned)((n + kRadixTile - 1) / kRadixTile); }
 
-// scratch: 256 * radix_tiles(n) uint32
-inline size_t radix_scratch_words(size_t n) { return (size_t)256 * radix_tiles(n) + 64; }
+// scratch: the (digit, tile) table, then 256 digit totals per pass
+inline size_t radix_scratch_words(size_t n) { return (size_t)256 * radix_tiles(n) + 256 * kMaxPasses + 64; }
 
-template <typename K>
+template <typename T>
+__global__ __launch_bounds__(kRadixBlock) void k_radix_zero(T *__restrict__ p, unsigned n) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+// FUSED: counts tile-major (tile * 256 + d) and added into totals; else digit-major (d * ntiles
+// + tile) for the scan kernel.
+template <typename K, bool FUSED>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *__restrict__ keys, size_t n, int shift,
-                                                            uint32_t *__restrict__ hist, unsigned ntiles) {
+                                                            uint32_t *__restrict__ hist, unsigned ntiles,
+                                                            uint32_t *__restrict__ totals) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -36,18 +53,51 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *__restrict_
         if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    const uint32_t c = h[threadIdx.x];
+    if (FUSED) {
+        hist[(size_t)blockIdx.x * 256 + threadIdx.x] = c;
+        if (c) atomicAdd(&totals[threadIdx.x], c);
+    } else {
+        hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = c;
+    }
+}
+
+// Scatter offset of digit threadIdx.x for tile blockIdx.x from the tile-major counts and the digit
+// totals: (exclusive scan of the totals) + (the digit's count in the earlier tiles).
+__device__ __forceinline__ uint32_t radix_fused_offset(const uint32_t *__restrict__ hist,
+                                                       const uint32_t *__restrict__ totals) {
+    __shared__ uint32_t wsum[4];
+    const unsigned d = threadIdx.x, lane = d & 63u, w = d >> 6;
+    const uint32_t v = totals[d];
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    uint32_t pre = 0;
+    const unsigned t = blockIdx.x;
+    unsigned i = 0;
+    for (; i + 4 <= t; i += 4)
+        pre += hist[(size_t)i * 256 + d] + hist[(size_t)(i + 1) * 256 + d] + hist[(size_t)(i + 2) * 256 + d] +
+               hist[(size_t)(i + 3) * 256 + d];
+    for (; i < t; i++) pre += hist[(size_t)i * 256 + d];
+    __syncthreads();
+    uint32_t base = x - v;
+    for (unsigned j = 0; j < w; j++) base += wsum[j];
+    return base + pre;
 }
 
 // Exclusive scan of m words in place (one block of 1024 threads; lbvh.hip).
 __global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m);
 
-template <typename K>
+template <typename K, bool FUSED>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *__restrict__ kin,
                                                                const uint32_t *__restrict__ vin,
                                                                K *__restrict__ kout, uint32_t *__restrict__ vout,
                                                                size_t n, int shift, const uint32_t *__restrict__ offs,
-                                                               unsigned ntiles) {
+                                                               unsigned ntiles, const uint32_t *__restrict__ totals) {
     __shared__ uint32_t cnt[4][256];
     __shared__ uint32_t base[4][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -79,7 +129,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *__restri
     __syncthreads();
     {
         const uint32_t d = threadIdx.x;
-        uint32_t run = offs[(size_t)d * ntiles + blockIdx.x];
+        uint32_t run = FUSED ? radix_fused_offset(offs, totals) : offs[(size_t)d * ntiles + blockIdx.x];
 #pragma unroll
         for (int ww = 0; ww < 4; ww++) {
             base[ww][d] = run;
@@ -112,11 +162,27 @@ hipError_t radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals
     uint32_t *vi = vals, *vo = vals_alt;
     if (n > 1) {
         const unsigned nt = radix_tiles(n);
-        for (int shift = 0; shift < key_bits; shift += 8) {
-            hipLaunchKernelGGL(k_radix_hist<K>, dim3(nt), dim3(kRadixBlock), 0, stream, ki, n, shift, scratch, nt);
-            hipLaunchKernelGGL(k_scan_exclusive_u32, dim3(1), dim3(1024), 0, stream, scratch, (size_t)256 * nt);
-            hipLaunchKernelGGL(k_radix_scatter<K>, dim3(nt), dim3(kRadixBlock), 0, stream, ki, vi, ko, vo, n, shift,
-                               scratch, nt);
+        const bool fused = nt <= kFuseTiles;
+        uint32_t *totals = scratch + (size_t)256 * nt;
+        const int passes = (key_bits + 7) / 8;
+        if (passes > kMaxPasses) return hipErrorInvalidValue;
+        if (fused)
+            hipLaunchKernelGGL(k_radix_zero<uint32_t>, dim3(passes), dim3(kRadixBlock), 0, stream, totals,
+                               (unsigned)(256 * passes));
+        for (int shift = 0, pass = 0; shift < key_bits; shift += 8, pass++) {
+            if (fused) {
+                uint32_t *tp = totals + 256 * pass;
+                hipLaunchKernelGGL((k_radix_hist<K, true>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, n, shift,
+                                   scratch, nt, tp);
+                hipLaunchKernelGGL((k_radix_scatter<K, true>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, vi, ko,
+                                   vo, n, shift, scratch, nt, tp);
+            } else {
+                hipLaunchKernelGGL((k_radix_hist<K, false>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, n, shift,
+                                   scratch, nt, (uint32_t *)nullptr);
+                hipLaunchKernelGGL(k_scan_exclusive_u32, dim3(1), dim3(1024), 0, stream, scratch, (size_t)256 * nt);
+                hipLaunchKernelGGL((k_radix_scatter<K, false>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, vi, ko,
+                                   vo, n, shift, scratch, nt, (const uint32_t *)nullptr);
+            }
             K *tk = ki;
             ki = ko, ko = tk;
             uint32_t *tv = vi;
