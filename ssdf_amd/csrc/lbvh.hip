@@ -8,9 +8,9 @@
 //   radix sort    (key, triangle) pairs, 8 passes of 8 bits, stable -> equal keys keep index order
 //   k_karras      internal node i's children and parent links (Karras, HPG 2012), ties in the keys
 //                 split by position, so the tree is a pure function of the mesh
-//   k_refit_pack  one thread per leaf: packs the Morton-ordered 144-byte TriP record and merges its
-//                 box into the ancestors' child slots with atomic min/max until a slot already
-//                 contains it; k_finalize_nodes turns the slots into the Nodes' float boxes
+//   k_refit_pack  one thread per leaf: packs the Morton-ordered 144-byte TriP record and refits the
+//                 boxes bottom-up, in LDS within the block's leaves and with atomic min/max above;
+//                 k_finalize_nodes turns the slots into the Nodes' float boxes
 //   k_depth       deepest leaf (sizes the traversal stack)
 // The same construction, run serially, is the oracle's canonical replay (oracle/oracle.c), so the
 // device tree and the replay's tree are identical.
@@ -329,20 +329,31 @@ __global__ __launch_bounds__(256) void k_slots_init(int *__restrict__ slots, siz
     if (i < nwords) slots[i] = ((i % 6) < 3) ? 0x7fffffff : (int)0x80000000;
 }
 
-// One thread per leaf (Morton position k): packs the TriP record, then climbs towards the root,
-// merging its box into each ancestor's child slot with atomic min/max, and stops at the first slot
-// that already contains it.  Stopping there is exact: each bound of that slot came from a leaf of
-// the same subtree, and that leaf carries the bound (or a tighter one) upward itself.  No fences
-// and no cross-XCD cache writebacks are needed; the next kernel sees every atomic's result.
+// One thread per leaf (Morton position k): packs the TriP record, then climbs towards the root.
+// While the node lies inside the block's 256 leaves, the boxes meet in LDS: each arriving thread
+// writes its side, and the second of the two merges both and climbs on (workgroup-scope ordering
+// only).  From the first node that reaches outside the block, the thread merges its box into each
+// ancestor's child slot with atomic min/max and stops at the first slot that already contains it.
+// Stopping there is exact: each bound of that slot came from a thread of the same subtree, and that
+// thread carries the bound (or a tighter one) upward itself.  No global fences and no cross-XCD
+// cache writebacks are needed; the next kernel sees every store's and atomic's result.  The LDS
+// stage takes the ~6 device-scope atomics per level off the lower levels, where most threads are.
 template <typename G>
 __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__ order, int n,
                                                     const int *__restrict__ s0, const int *__restrict__ s1,
                                                     const int *__restrict__ s2, const G *__restrict__ sx,
                                                     const G *__restrict__ sy, const G *__restrict__ sz,
                                                     const float *__restrict__ boxes, const int *__restrict__ parent,
-                                                    int *__restrict__ slots, TriP *__restrict__ tris,
-                                                    MeshMeta *__restrict__ meta) {
+                                                    const Node *__restrict__ nodes, int *__restrict__ slots,
+                                                    TriP *__restrict__ tris, MeshMeta *__restrict__ meta) {
+    // child boxes and arrival counts of the internal nodes whose leaf range lies inside this
+    // block's 256 leaves (such a node's index is one end of its range, so it is inside too)
+    __shared__ int lbox[256][12];
+    __shared__ int lcnt[256];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b0 = blockIdx.x * blockDim.x, b1 = min(n - 1, b0 + (int)blockDim.x - 1);
+    lcnt[threadIdx.x] = 0;
+    __syncthreads();
     if (k >= n) return;
     const int t = (int)order[k];
     {
@@ -364,16 +375,39 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
 #pragma unroll
     for (int d = 0; d < 6; d++) b[d] = float_to_ordered(boxes[(size_t)6 * t + d]);
     int p = parent[n - 1 + k];
+    bool local = true;
     for (int guard = 0; guard < 4096; guard++) {
         const int pi = p >> 1, side = p & 1;
-        int *sl = slots + (size_t)12 * pi + 6 * side;
-        bool grew = false;
+        if (local) local = nodes[pi].first >= b0 && nodes[pi].last <= b1;  // ranges nest: once out, out
+        if (local) {
+            // inside the block: LDS, and the second of the two threads to arrive merges and climbs
+            int *ls = lbox[pi - b0];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            grew |= atomicMin(sl + d, b[d]) > b[d];
-            grew |= atomicMax(sl + 3 + d, b[3 + d]) < b[3 + d];
+            for (int d = 0; d < 6; d++) ls[6 * side + d] = b[d];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (atomicAdd(&lcnt[pi - b0], 1) == 0) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int *os = ls + 6 * (side ^ 1);
+            int *gs = slots + (size_t)12 * pi;  // the node's final child boxes, for k_finalize_nodes
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                const int o = os[d];
+                gs[6 * side + d] = b[d];
+                gs[6 * (side ^ 1) + d] = o;
+                b[d] = d < 3 ? min(b[d], o) : max(b[d], o);
+            }
+        } else {
+            // across blocks: atomic min/max into the child slot, climbing while the slot grew
+            int *sl = slots + (size_t)12 * pi + 6 * side;
+            bool grew = false;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                grew |= atomicMin(sl + d, b[d]) > b[d];
+                grew |= atomicMax(sl + 3 + d, b[3 + d]) < b[3 + d];
+            }
+            if (!grew) break;
         }
-        if (!grew || pi == 0) break;
+        if (pi == 0) break;
         p = parent[pi];
     }
 }
@@ -468,12 +502,12 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
     }
     if (m.dtype == 0)
         hipLaunchKernelGGL(k_refit_pack<float>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
-                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, slots, m.tris,
-                           m.meta);
+                           (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, m.nodes,
+                           slots, m.tris, m.meta);
     else
         hipLaunchKernelGGL(k_refit_pack<double>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
-                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent, slots,
-                           m.tris, m.meta);
+                           (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent,
+                           m.nodes, slots, m.tris, m.meta);
     if (n > 1) {
         hipLaunchKernelGGL(k_finalize_nodes, dim3(nblk(ni, 256)), dim3(256), 0, stream, slots, (int)ni, m.nodes,
                            m.meta);
