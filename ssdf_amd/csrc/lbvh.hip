@@ -57,56 +57,6 @@ int ensure_ws(DevMesh &m, size_t bytes, hipStream_t stream) {
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *get_error() { return g_last_error.c_str(); }
 
-// Exclusive scan of m words in place, one block of 1024 threads.  Rounds of up to 16 K words:
-// each thread scans its own run of up to 16 consecutive words in registers, the 1024 run totals
-// are scanned by wave shuffles plus a scan of the 16 wave totals, and a carry links the rounds.
-__global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m) {
-    constexpr int kPer = 16;
-    __shared__ uint32_t wsum[16];
-    const unsigned tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    uint32_t carry = 0;
-    for (size_t base = 0; base < m; base += (size_t)1024 * kPer) {
-        const size_t len = m - base < (size_t)1024 * kPer ? m - base : (size_t)1024 * kPer;
-        const size_t per = (len + 1023) / 1024;
-        const size_t s0 = base + (size_t)tid * per;
-        uint32_t v[kPer];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const size_t i = s0 + (size_t)k;
-            v[k] = ((size_t)k < per && i < base + len) ? data[i] : 0u;
-            sum += v[k];
-        }
-        uint32_t x = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if ((int)lane >= o) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        if (w == 0) {
-            uint32_t t = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                const uint32_t y = __shfl_up(t, o, 64);
-                if ((int)lane >= o) t += y;
-            }
-            if (lane < 16) wsum[lane] = t;
-        }
-        __syncthreads();
-        uint32_t run = carry + (w ? wsum[w - 1] : 0u) + x - sum;
-#pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const size_t i = s0 + (size_t)k;
-            if ((size_t)k < per && i < base + len) data[i] = run;
-            run += v[k];
-        }
-        carry += wsum[15];
-        __syncthreads();
-    }
-}
-
 __global__ void k_mesh_init(MeshMeta *meta) {
     if (threadIdx.x == 0) {
         meta->root = 0;

@@ -9,7 +9,8 @@
 // Up to kFuseTiles tiles the scan lives inside the other two kernels: hist writes its counts
 // tile-major and adds them into the pass's digit totals (zeroed once per sort), and each scatter
 // block scans the 256 totals and sums the earlier tiles' counts itself (coalesced, L2-resident).
-// Larger sorts take a third, one-block scan kernel over a digit-major table.
+// Larger sorts take a third kernel between them over a digit-major table: one block per digit
+// scans that digit's row of tile counts, offset by the totals of the lower digits.
 // Tiles are 4096 keys (256 threads x 16).  All launches go to the caller's stream and nothing is
 // read back, so a sort can sit inside a captured graph.
 #pragma once
@@ -54,11 +55,49 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const K *__restrict_
     }
     __syncthreads();
     const uint32_t c = h[threadIdx.x];
-    if (FUSED) {
+    if (FUSED)
         hist[(size_t)blockIdx.x * 256 + threadIdx.x] = c;
-        if (c) atomicAdd(&totals[threadIdx.x], c);
-    } else {
+    else
         hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = c;
+    if (c) atomicAdd(&totals[threadIdx.x], c);
+}
+
+// Inclusive scan of one value per thread over a 256-thread block (x: this thread's value).
+__device__ __forceinline__ uint32_t radix_block_scan256(uint32_t x, uint32_t *wsum) {
+    const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    for (unsigned j = 0; j < w; j++) x += wsum[j];
+    __syncthreads();
+    return x;
+}
+
+// Digit-major table -> scatter offsets, block d scanning row d (ntiles counts) in place.
+template <typename T>
+__global__ __launch_bounds__(kRadixBlock) void k_radix_scan_rows(T *__restrict__ hist, unsigned ntiles,
+                                                                 const T *__restrict__ totals) {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    const unsigned d = blockIdx.x, tid = threadIdx.x;
+    // offset of digit d: the totals of the digits below it (thread 255 ends with their sum)
+    const uint32_t lower = radix_block_scan256(tid < d ? totals[tid] : 0u, wsum);
+    if (tid == 255) carry = lower;
+    __syncthreads();
+    uint32_t *row = hist + (size_t)d * ntiles;
+    for (unsigned b = 0; b < ntiles; b += kRadixBlock) {
+        const unsigned i = b + tid;
+        const uint32_t v = i < ntiles ? row[i] : 0u;
+        const uint32_t inc = radix_block_scan256(v, wsum);
+        const uint32_t c = carry;
+        if (i < ntiles) row[i] = c + inc - v;
+        __syncthreads();
+        if (tid == 255) carry = c + inc;
+        __syncthreads();
     }
 }
 
@@ -88,9 +127,6 @@ __device__ __forceinline__ uint32_t radix_fused_offset(const uint32_t *__restric
     for (unsigned j = 0; j < w; j++) base += wsum[j];
     return base + pre;
 }
-
-// Exclusive scan of m words in place (one block of 1024 threads; lbvh.hip).
-__global__ __launch_bounds__(1024) void k_scan_exclusive_u32(uint32_t *__restrict__ data, size_t m);
 
 template <typename K, bool FUSED>
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const K *__restrict__ kin,
@@ -166,9 +202,8 @@ hipError_t radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals
         uint32_t *totals = scratch + (size_t)256 * nt;
         const int passes = (key_bits + 7) / 8;
         if (passes > kMaxPasses) return hipErrorInvalidValue;
-        if (fused)
-            hipLaunchKernelGGL(k_radix_zero<uint32_t>, dim3(passes), dim3(kRadixBlock), 0, stream, totals,
-                               (unsigned)(256 * passes));
+        hipLaunchKernelGGL(k_radix_zero<uint32_t>, dim3(passes), dim3(kRadixBlock), 0, stream, totals,
+                           (unsigned)(256 * passes));
         for (int shift = 0, pass = 0; shift < key_bits; shift += 8, pass++) {
             if (fused) {
                 uint32_t *tp = totals + 256 * pass;
@@ -177,9 +212,11 @@ hipError_t radix_sort_pairs(K *keys, K *keys_alt, uint32_t *vals, uint32_t *vals
                 hipLaunchKernelGGL((k_radix_scatter<K, true>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, vi, ko,
                                    vo, n, shift, scratch, nt, tp);
             } else {
+                uint32_t *tp = totals + 256 * pass;
                 hipLaunchKernelGGL((k_radix_hist<K, false>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, n, shift,
-                                   scratch, nt, (uint32_t *)nullptr);
-                hipLaunchKernelGGL(k_scan_exclusive_u32, dim3(1), dim3(1024), 0, stream, scratch, (size_t)256 * nt);
+                                   scratch, nt, tp);
+                hipLaunchKernelGGL(k_radix_scan_rows<uint32_t>, dim3(256), dim3(kRadixBlock), 0, stream, scratch, nt,
+                                   tp);
                 hipLaunchKernelGGL((k_radix_scatter<K, false>), dim3(nt), dim3(kRadixBlock), 0, stream, ki, vi, ko,
                                    vo, n, shift, scratch, nt, (const uint32_t *)nullptr);
             }
