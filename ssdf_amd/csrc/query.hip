@@ -25,11 +25,13 @@
 namespace ssdf_amd {
 
 constexpr int kWaves = 4;
-// FIX candidate list: 16 KB of LDS per block, so that LDS allows 8 waves per SIMD (the packet
-// kernels ask the compiler for 8: 64 VGPRs).  Measured on the 200 k-triangle torus at 256^3: 16
-// entries and 8 waves take 7.7 ms, 24 entries and 6 waves 8.4 ms; a query with more than 16
-// triangles in its band goes to the per-query kernel.
-constexpr int kCand = 16;
+// FIX candidate list: 20 KB of LDS per block (5 KB per wave), the most that lets LDS allow 8 waves
+// per SIMD (the packet kernels ask the compiler for 8: 64 VGPRs).  A query with more than kCand
+// triangles in its band goes to the per-query kernel, which re-walks the tree in FP64.  Measured at
+// 256^3 (FP32 + fixup): 200 k-triangle torus 7.9 / 7.8 / 8.4 ms with 16 / 20 / 24 entries; 2 M-
+// triangle bumpy sphere 415 / 212 / 207 ms (1.5 M / 0.68 M / 0.32 M queries queued: inside a fine
+// near-sphere, 17-24 triangles in a band are common).
+constexpr int kCand = 20;
 constexpr int kNone = INT_MIN;
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
