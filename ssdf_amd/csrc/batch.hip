@@ -31,8 +31,16 @@ struct ssdf_batch {
     TriP *tris = nullptr;
     MeshMeta *metas = nullptr;
     double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
-    char *ws = nullptr;
+    char *ws = nullptr;  // build workspace of lane 0
     size_t ws_cap = 0;
+    // Build lanes: meshes are built concurrently on up to kLanes streams (lane 0: the caller's),
+    // each lane with a workspace of its own; events fork the lanes off the caller's stream and
+    // join them back (in a captured graph too).
+    static constexpr int kLanes = 4;
+    hipStream_t lane_st[kLanes] = {};
+    hipEvent_t lane_ev[kLanes] = {};  // [0]: fork; [l]: join of lane l
+    char *lane_ws[kLanes] = {};       // lanes 1..
+    size_t lane_ws_cap[kLanes] = {};
     size_t device_bytes = 0;
     // query state
     BatchMesh *d_desc = nullptr, *h_desc = nullptr;  // device / pinned host descriptors
@@ -63,8 +71,13 @@ void destroy_batch(ssdf_batch *b) {
                     (void *)b->vertex_n, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
                     (void *)b->fix_list})
         dev_free(p, st);
+    for (int l = 1; l < ssdf_batch::kLanes; l++) dev_free(b->lane_ws[l], st);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
     (void)hipStreamSynchronize(st);
+    for (int l = 0; l < ssdf_batch::kLanes; l++) {
+        if (b->lane_st[l]) (void)hipStreamDestroy(b->lane_st[l]);
+        if (b->lane_ev[l]) (void)hipEventDestroy(b->lane_ev[l]);
+    }
     delete b;
 }
 
@@ -75,14 +88,54 @@ int arena(T **p, size_t n, ssdf_batch *b) {
     return SSDF_OK;
 }
 
-// Builds mesh i (LBVH, and normals) with the batch's shared workspace.
-int build_one(ssdf_batch *b, int i, bool normals, hipStream_t st) {
+// Builds mesh i (LBVH, and normals) on stream st with lane l's workspace.
+int build_one(ssdf_batch *b, int i, int l, bool normals, hipStream_t st) {
+    char *&ws = l ? b->lane_ws[l] : b->ws;
+    size_t &cap = l ? b->lane_ws_cap[l] : b->ws_cap;
     DevMesh &m = b->meshes[i];
-    m.ws = b->ws, m.ws_cap = b->ws_cap;
+    m.ws = ws, m.ws_cap = cap;
     int rc = build_lbvh(m, st);
     if (!rc && normals) rc = build_normals(m, st);
-    b->ws = m.ws, b->ws_cap = m.ws_cap;  // grown if it had to
+    ws = m.ws, cap = m.ws_cap;  // grown if it had to (only before any capture: the same meshes
+                                // land on the same lanes every time)
     m.ws = nullptr, m.ws_cap = 0;
+    return rc;
+}
+
+// Builds every mesh, the lanes running concurrently: largest mesh first onto the least loaded
+// lane.  Lane l > 0 runs on the batch's own stream, forked off st and joined back into it.
+int build_lanes(ssdf_batch *b, bool normals, hipStream_t st) {
+    constexpr int L = ssdf_batch::kLanes;
+    for (int l = 0; l < L; l++) {
+        if (!b->lane_ev[l]) SSDF_HIP_TRY(hipEventCreateWithFlags(&b->lane_ev[l], hipEventDisableTiming));
+        if (l && !b->lane_st[l]) SSDF_HIP_TRY(hipStreamCreateWithFlags(&b->lane_st[l], hipStreamNonBlocking));
+    }
+    std::vector<int> order(b->nm);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->meshes[x].nsel > b->meshes[y].nsel; });
+    std::vector<int> lane_of(b->nm, 0);
+    long long load[L] = {};
+    for (int i : order) {
+        int l = 0;
+        for (int k = 1; k < L; k++)
+            if (load[k] < load[l]) l = k;
+        lane_of[i] = l;
+        load[l] += std::max<long long>(b->meshes[i].nsel, 0);
+    }
+    SSDF_HIP_TRY(hipEventRecord(b->lane_ev[0], st));
+    for (int l = 1; l < L; l++) SSDF_HIP_TRY(hipStreamWaitEvent(b->lane_st[l], b->lane_ev[0], 0));
+    int rc = SSDF_OK;
+    for (int i : order)
+        if (!rc && b->meshes[i].nsel > 0) {
+            const int l = lane_of[i];
+            rc = build_one(b, i, l, normals, l ? b->lane_st[l] : st);
+        }
+    // join every lane, also after an error (a capture must end with its streams joined)
+    for (int l = 1; l < L; l++) {
+        SSDF_HIP_TRY(hipEventRecord(b->lane_ev[l], b->lane_st[l]));
+        SSDF_HIP_TRY(hipStreamWaitEvent(st, b->lane_ev[l], 0));
+    }
     return rc;
 }
 
@@ -104,14 +157,7 @@ int sync_metas(ssdf_batch *b, hipStream_t st) {
 }
 
 int rebuild_all(ssdf_batch *b, bool normals, hipStream_t st) {
-    // the largest mesh first, so that the shared workspace grows once
-    std::vector<int> order(b->nm);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return b->meshes[x].nsel > b->meshes[y].nsel; });
-    for (int i : order)
-        if (b->meshes[i].nsel > 0)
-            if (int rc = build_one(b, i, normals, st)) return rc;
+    if (int rc = build_lanes(b, normals, st)) return rc;
     for (int i = 0; i < b->nm; i++) b->meshes[i].has_normals = normals && b->meshes[i].nsel > 0;
     return SSDF_OK;
 }
@@ -389,6 +435,7 @@ int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t
     if (int rc = rebuild_all(b, b->with_normals, st)) return rc;
     if (int rc = sync_metas(b, st)) return rc;
     b->device_bytes += b->ws_cap;
+    for (int l = 1; l < ssdf_batch::kLanes; l++) b->device_bytes += b->lane_ws_cap[l];
     *out = b;
     guard.b = nullptr;
     return SSDF_OK;
@@ -520,18 +567,7 @@ int ssdf_batch_capture(ssdf_batch_t b, int rebuild, int mode, const ptrdiff_t *d
     const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
     SSDF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = SSDF_OK;
-    if (rebuild) {
-        std::vector<int> order(b->nm);
-        std::iota(order.begin(), order.end(), 0);
-        for (int i : order)
-            if (!rc && b->meshes[i].nsel > 0) {
-                DevMesh &m = b->meshes[i];
-                m.ws = b->ws, m.ws_cap = b->ws_cap;  // large enough since creation: no allocation
-                rc = build_lbvh(m, st);
-                if (!rc && b->with_normals) rc = build_normals(m, st);
-                m.ws = nullptr, m.ws_cap = 0;
-            }
-    }
+    if (rebuild) rc = build_lanes(b, b->with_normals, st);  // workspaces sized since creation
     if (!rc && p.total_units > 0) rc = launch_batch(a, b->d_desc, QSRC_GRID, mode, fix, stack,
                                                     grid_blocks(b, p.total_units, fix), st);
     hipGraph_t g = nullptr;
