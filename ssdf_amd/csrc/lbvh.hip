@@ -9,9 +9,9 @@
 //   k_karras      internal node i's children and parent links (Karras, HPG 2012), ties in the keys
 //                 split by position, so the tree is a pure function of the mesh
 //   k_refit_pack  one thread per leaf: packs the Morton-ordered 144-byte TriP record and refits the
-//                 boxes bottom-up, in LDS within the block's leaves and with atomic min/max above;
-//                 k_finalize_nodes turns the slots into the Nodes' float boxes
-//   k_depth       deepest leaf (sizes the traversal stack)
+//                 boxes and subtree heights bottom-up, in LDS within the block's leaves and with
+//                 atomic min/max above; k_finalize_nodes turns the slots into the Nodes' float
+//                 boxes and the root's height into the tree depth (sizes the traversal stack)
 // The same construction, run serially, is the oracle's canonical replay (oracle/oracle.c), so the
 // device tree and the replay's tree are identical.
 #include "mesh.hpp"
@@ -274,9 +274,11 @@ __device__ __forceinline__ float to_f_near(G v) {
 
 // Child-slot boxes during the build, as ordered ints (12 per internal node: left min xyz, max xyz,
 // right min xyz, max xyz), so that min/max merge with integer atomics.
-__global__ __launch_bounds__(256) void k_slots_init(int *__restrict__ slots, size_t nwords) {
+__global__ __launch_bounds__(256) void k_slots_init(int *__restrict__ slots, size_t nwords, int *__restrict__ hts,
+                                                    size_t nh) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nwords) slots[i] = ((i % 6) < 3) ? 0x7fffffff : (int)0x80000000;
+    if (i < nh) hts[i] = 0;
 }
 
 // One thread per leaf (Morton position k): packs the TriP record, then climbs towards the root.
@@ -295,10 +297,12 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
                                                     const G *__restrict__ sy, const G *__restrict__ sz,
                                                     const float *__restrict__ boxes, const int *__restrict__ parent,
                                                     const Node *__restrict__ nodes, int *__restrict__ slots,
-                                                    TriP *__restrict__ tris, MeshMeta *__restrict__ meta) {
+                                                    int *__restrict__ hts, TriP *__restrict__ tris,
+                                                    MeshMeta *__restrict__ meta) {
     // child boxes and arrival counts of the internal nodes whose leaf range lies inside this
     // block's 256 leaves (such a node's index is one end of its range, so it is inside too)
     __shared__ int lbox[256][12];
+    __shared__ int lht[256][2];
     __shared__ int lcnt[256];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int b0 = blockIdx.x * blockDim.x, b1 = min(n - 1, b0 + (int)blockDim.x - 1);
@@ -324,7 +328,10 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
     int b[6];
 #pragma unroll
     for (int d = 0; d < 6; d++) b[d] = float_to_ordered(boxes[(size_t)6 * t + d]);
+    // h: height of the subtree this thread carries (a leaf: 0; an internal node: 1 + its
+    // children's larger height); in the atomic part a lower bound, merged with atomic max
     int p = parent[n - 1 + k];
+    int h = 0;
     bool local = true;
     for (int guard = 0; guard < 4096; guard++) {
         const int pi = p >> 1, side = p & 1;
@@ -334,6 +341,7 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
             int *ls = lbox[pi - b0];
 #pragma unroll
             for (int d = 0; d < 6; d++) ls[6 * side + d] = b[d];
+            lht[pi - b0][side] = h;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (atomicAdd(&lcnt[pi - b0], 1) == 0) break;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -346,16 +354,21 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
                 gs[6 * (side ^ 1) + d] = o;
                 b[d] = d < 3 ? min(b[d], o) : max(b[d], o);
             }
+            const int ho = lht[pi - b0][side ^ 1];
+            hts[2 * pi + side] = h;
+            hts[2 * pi + (side ^ 1)] = ho;
+            h = 1 + max(h, ho);
         } else {
             // across blocks: atomic min/max into the child slot, climbing while the slot grew
             int *sl = slots + (size_t)12 * pi + 6 * side;
-            bool grew = false;
+            bool grew = atomicMax(hts + 2 * pi + side, h) < h;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 grew |= atomicMin(sl + d, b[d]) > b[d];
                 grew |= atomicMax(sl + 3 + d, b[3 + d]) < b[3 + d];
             }
             if (!grew) break;
+            h += 1;
         }
         if (pi == 0) break;
         p = parent[pi];
@@ -363,8 +376,9 @@ __global__ __launch_bounds__(256) void k_refit_pack(const uint32_t *__restrict__
 }
 
 // Ordered-int slots -> the float child boxes of each Node; the root's box into the meta.
-__global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ slots, int ninternal,
-                                                        Node *__restrict__ nodes, MeshMeta *__restrict__ meta) {
+__global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ slots, const int *__restrict__ hts,
+                                                        int ninternal, Node *__restrict__ nodes,
+                                                        MeshMeta *__restrict__ meta) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninternal) return;
     const int *sl = slots + (size_t)12 * i;
@@ -377,29 +391,12 @@ __global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ 
         nd->rmax[d] = ordered_to_float(sl[9 + d]);
     }
     if (i == 0) {
+        meta->maxdepth = 1 + max(hts[0], hts[1]);  // leaves below the deepest path: internal nodes on it
         for (int d = 0; d < 3; d++) {
             meta->root_box[d] = fminf(ordered_to_float(sl[d]), ordered_to_float(sl[6 + d]));
             meta->root_box[3 + d] = fmaxf(ordered_to_float(sl[3 + d]), ordered_to_float(sl[9 + d]));
         }
     }
-}
-
-__global__ __launch_bounds__(256) void k_depth(int n, const int *__restrict__ parent, MeshMeta *__restrict__ meta) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || n < 2) return;
-    int d = 0;
-    int p = parent[n - 1 + k];
-    for (;;) {
-        d++;
-        const int pi = p >> 1;
-        if (pi == 0) break;
-        if (d >= 4096) {  // not a tree: report an impossible depth instead of spinning
-            d = 1 << 20;
-            break;
-        }
-        p = parent[pi];
-    }
-    atomicMax(&meta->maxdepth, d);
 }
 
 static unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -417,11 +414,12 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
         SSDF_HIP_TRY(dev_alloc((void **)&m.tris, sizeof(TriP) * nn, stream));
         m.device_bytes += sizeof(Node) * ni + sizeof(TriP) * nn + sizeof(MeshMeta);
     }
-    // one workspace allocation: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, slots 48 B per
-    // triangle, block partials, radix scratch
+    // one workspace allocation: boxes 24 B, keys 2 x 8 B, vals 2 x 4 B, parent 8 B, slots 48 B and
+    // child heights 8 B per triangle, block partials, radix scratch
     const unsigned nb = nblk(nn, 256);
     const size_t off_keys = 0, off_vals = off_keys + 16 * nn, off_boxes = off_vals + 8 * nn,
-                 off_parent = off_boxes + 24 * nn, off_slots = off_parent + 8 * nn, off_part = off_slots + 48 * ni,
+                 off_parent = off_boxes + 24 * nn, off_slots = off_parent + 8 * nn, off_hts = off_slots + 48 * ni,
+                 off_part = off_hts + 8 * ni,
                  off_scr = off_part + 48 * (size_t)nb, total = off_scr + 4 * radix_scratch_words(nn);
     if (int rc = ensure_ws(m, total, stream)) return rc;
     char *ws = m.ws;
@@ -430,6 +428,7 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
     float *boxes = (float *)(ws + off_boxes);
     int *parent = (int *)(ws + off_parent);
     int *slots = (int *)(ws + off_slots);
+    int *hts = (int *)(ws + off_hts);
     float *partials = (float *)(ws + off_part);
     uint32_t *scratch = (uint32_t *)(ws + off_scr);
 
@@ -448,20 +447,19 @@ int build_lbvh(DevMesh &m, hipStream_t stream) {
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(nblk(nn - 1, 256)), dim3(256), 0, stream, skeys, n, m.nodes, parent,
                            (int *)nullptr);
-        hipLaunchKernelGGL(k_slots_init, dim3(nblk(12 * ni, 256)), dim3(256), 0, stream, slots, 12 * ni);
+        hipLaunchKernelGGL(k_slots_init, dim3(nblk(12 * ni, 256)), dim3(256), 0, stream, slots, 12 * ni, hts, 2 * ni);
     }
     if (m.dtype == 0)
         hipLaunchKernelGGL(k_refit_pack<float>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
                            (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, boxes, parent, m.nodes,
-                           slots, m.tris, m.meta);
+                           slots, hts, m.tris, m.meta);
     else
         hipLaunchKernelGGL(k_refit_pack<double>, dim3(nb), dim3(256), 0, stream, svals, n, m.s0, m.s1, m.s2,
                            (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, boxes, parent,
-                           m.nodes, slots, m.tris, m.meta);
+                           m.nodes, slots, hts, m.tris, m.meta);
     if (n > 1) {
-        hipLaunchKernelGGL(k_finalize_nodes, dim3(nblk(ni, 256)), dim3(256), 0, stream, slots, (int)ni, m.nodes,
+        hipLaunchKernelGGL(k_finalize_nodes, dim3(nblk(ni, 256)), dim3(256), 0, stream, slots, hts, (int)ni, m.nodes,
                            m.meta);
-        hipLaunchKernelGGL(k_depth, dim3(nb), dim3(256), 0, stream, n, parent, m.meta);
     }
     SSDF_HIP_TRY(hipGetLastError());
     return SSDF_OK;
