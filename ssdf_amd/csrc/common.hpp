@@ -185,6 +185,8 @@ __host__ __device__ __forceinline__ T pt_box_d2(const T px, const T py, const T 
 
 // FP32 point-AABB squared distance for pruning: per axis max(lo - p, p - hi, 0) (one v_max3),
 // sum with fused multiply-adds.  Same value as pt_box_d2 up to the rounding of the final sum.
+// (p - med3(p, lo, hi) would be one instruction shorter, but a VOP3 reads one SGPR only, and both
+// bounds arrive in SGPRs from the scalar node load: the compiler adds a v_mov per axis.)
 __host__ __device__ __forceinline__ float box_d2_fast(const float px, const float py, const float pz, const float *mn,
                                              const float *mx) {
     const float dx = fmaxf(fmaxf(mn[0] - px, px - mx[0]), 0.0f);

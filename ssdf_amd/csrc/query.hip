@@ -242,18 +242,17 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
             const uint64_t mr = __ballot(FIX ? dr <= thr : dr < best);
             const uint64_t mle = __ballot(dl <= dr);
-            const bool both = (ml != 0) & (mr != 0);
-            const bool lfirst = 2 * __popcll(ml & (~mr | mle)) >= __popcll(ml | mr);
-            const int ent = (node << 1) | (lfirst ? 1 : 0);
-            if (STACK == 64) {
-                stackv = (both && lane == sp) ? ent : stackv;
-            } else {
-                stackv = (both && lane == sp) ? ent : stackv;
-                stackv_hi = (both && lane == sp - 64) ? ent : stackv_hi;
-            }
-            sp += both ? 1 : 0;
-            node = uniform_i(both ? (lfirst ? nd.left : nd.right)
-                                  : (ml != 0 ? nd.left : (mr != 0 ? nd.right : kNone)));
+            const uint64_t many = ml | mr;
+            // the vote also picks a child that is alone in it: left alone wins 2|ml| >= |ml|,
+            // right alone loses 0 >= |mr|
+            const int lfirst = 2 * __popcll(ml & (~mr | mle)) >= __popcll(many) ? 1 : 0;
+            const int ent = (node << 1) | lfirst;  // the other child
+            // written at depth sp whether pushed or not: above the top it is overwritten before
+            // any pop reads it
+            stackv = lane == sp ? ent : stackv;
+            if (STACK == 128) stackv_hi = lane == sp - 64 ? ent : stackv_hi;
+            sp += (ml != 0 && mr != 0) ? 1 : 0;
+            node = uniform_i(many == 0 ? kNone : (lfirst ? nd.left : nd.right));
         }
         overflow |= sp > STACK;
         if (node != kNone) {

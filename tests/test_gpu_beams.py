@@ -23,7 +23,7 @@ def _grid(m, n, flags):
 
 
 def test_beams_closed_against_oracle(oracle):
-    m = W.beams(10, seed=21, n_circ=24, axial_per_half_unit=40)
+    m = W.beams(10, seed=21, n_circ=24, n_axial=40)
     n = 40
     o, t, info = _grid(m, n, S.FLAG_FIXUP)
     assert info["open_edges"] == 0
@@ -37,8 +37,8 @@ def test_beams_closed_against_oracle(oracle):
 
 
 def test_beams_holed_gwn_signs(oracle):
-    full = W.beams(10, seed=21, n_circ=24, axial_per_half_unit=40)
-    holed = W.beams(10, seed=21, n_circ=24, axial_per_half_unit=40, hole_fraction=0.01)
+    full = W.beams(10, seed=21, n_circ=24, n_axial=40)
+    holed = W.beams(10, seed=21, n_circ=24, n_axial=40, hole_fraction=0.01)
     n = 40
     a, _, _ = _grid(full, n, S.FLAG_FIXUP)
     b, _, info = _grid(holed, n, S.FLAG_FIXUP | S.FLAG_SIGN_AUTO)
