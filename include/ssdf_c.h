@@ -326,6 +326,11 @@ int ssdf_event_elapsed_ms(void *start, void *stop, float *ms);
 int ssdf_mesh_brute_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const float *x, const float *y, const float *z,
                            double *out_d2, int *out_tri, void *stream);
 
+/* LBVH leaf order (test reference): the original triangle index of each leaf, in leaf order, into
+ * the device array out_tri[nselements].  Leaves are the triangles stably sorted by the 63-bit
+ * Morton code of their box centre, so this exposes the build's radix sort to tests. */
+int ssdf_mesh_leaf_order(ssdf_mesh_t mesh, int *out_tri, void *stream);
+
 /* Leaf budget of a packet wave (no counterpart in the reference; a tuning knob of this library).
  * A wave of 64 queries that has evaluated this many leaves hands its unfinished queries to a
  * per-query kernel (one wave per query) with the bound found so far; results are unchanged (FP32 +

@@ -459,6 +459,26 @@ void orc_lbvh_counts(ptrdiff_t npoints, const float *x, const float *y, const fl
     orc_lbvh_free(&b);
 }
 
+/* Leaf order of the canonical LBVH: the triangle index of each leaf, i.e. the triangles stably
+ * sorted by the Morton code of their FP32 box centre (the order a correct GPU sort must produce). */
+void orc_lbvh_order(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
+                    const float *sz, int *out_order) {
+    float *fmin_ = (float *)calloc(3 * nsel + 1, sizeof(float)), *fmax_ = (float *)calloc(3 * nsel + 1, sizeof(float));
+    for (ptrdiff_t t = 0; t < nsel; t++) {
+        const int iv[3] = {s0[t], s1[t], s2[t]};
+        const float *co[3] = {sx, sy, sz};
+        for (int d = 0; d < 3; d++) {
+            const float a = co[d][iv[0]], b_ = co[d][iv[1]], c = co[d][iv[2]];
+            fmin_[3 * t + d] = ORC_MIN3(a, b_, c), fmax_[3 * t + d] = ORC_MAX3(a, b_, c);
+        }
+    }
+    orc_lbvh b;
+    orc_lbvh_build(&b, nsel, fmin_, fmax_);
+    for (ptrdiff_t k = 0; k < nsel; k++) out_order[k] = b.order[k];
+    free(fmin_), free(fmax_);
+    orc_lbvh_free(&b);
+}
+
 /* ================================================================================================
  * Pseudonormals; follows serial/bvh/bvh.cpp:1669-1721.  The reference pairs a face edge with the
  * first face seen on the same undirected edge through a hash map keyed min*V + max (1707-1719); a

@@ -99,6 +99,10 @@ void orc_lbvh_counts(ptrdiff_t npoints, const float *x, const float *y, const fl
                      const int *s0, const int *s1, const int *s2, const float *sx, const float *sy, const float *sz,
                      int64_t *out_nbox, int64_t *out_ntri, int *out_maxdepth, float *out_d2);
 
+/* Leaf order of the canonical LBVH (triangle index per leaf: stable sort by Morton code). */
+void orc_lbvh_order(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
+                    const float *sz, int *out_order);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

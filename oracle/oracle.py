@@ -45,6 +45,7 @@ def _load():
             "orc_sign": ([_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_gwn": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_lbvh_counts": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+            "orc_lbvh_order": ([_PD, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_num_threads": ([], C.c_int),
         }
         for name, (args, res) in sig.items():
@@ -263,3 +264,13 @@ def lbvh_counts(x, y, z, mesh):
     L.orc_lbvh_counts(n, _p(x), _p(y), _p(z), s0.size, _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), _p(nb),
                       _p(nt), C.byref(md), _p(d2))
     return nb, nt, int(md.value), d2
+
+
+def lbvh_order(mesh):
+    """Leaf order of the canonical LBVH: triangle index per leaf (stable sort by Morton code)."""
+    L = _load()
+    sx, sy, sz = [np.ascontiguousarray(v, dtype=np.float32) for v in (mesh.sx, mesh.sy, mesh.sz)]
+    s0, s1, s2 = _i32(mesh.s0, mesh.s1, mesh.s2)
+    out = np.empty(s0.size, np.int32)
+    L.orc_lbvh_order(s0.size, _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), _p(out))
+    return out

@@ -92,6 +92,7 @@ _SIGS = {
                              C.c_double, C.c_double, _P, _P, C.c_uint, _P],
     "ssdf_mesh_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
     "ssdf_mesh_brute_points": [_P, _PD, _P, _P, _P, _P, _P, _P],
+    "ssdf_mesh_leaf_order": [_P, _P, _P],
     "ssdf_batch_create": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
     "ssdf_batch_destroy": [_P],
     "ssdf_batch_rebuild": [_P, C.c_int, _P],
@@ -497,6 +498,11 @@ class ResidentMesh:
         rc = lib().ssdf_mesh_brute_points(C.c_void_p(self.h), x.n, _ptr(x), _ptr(y), _ptr(z), _ptr(out_d2),
                                           _ptr(out_tri), C.c_void_p(stream.ptr) if stream else None)
         _check(rc, "ssdf_mesh_brute_points")
+
+    def leaf_order(self, out_tri, stream: Stream | None = None):
+        """Original triangle index of each LBVH leaf, in leaf (Morton) order, into a device array."""
+        _check(lib().ssdf_mesh_leaf_order(C.c_void_p(self.h), _ptr(out_tri),
+                                          C.c_void_p(stream.ptr) if stream else None), "ssdf_mesh_leaf_order")
 
     def rebuild(self, with_normals: bool = True, stream: Stream | None = None):
         _check(lib().ssdf_mesh_rebuild(C.c_void_p(self.h), 1 if with_normals else 0,

@@ -87,6 +87,12 @@ __global__ __launch_bounds__(256) void k_brute_f64(const TriP *__restrict__ tris
     }
 }
 
+// Original triangle index of each LBVH leaf, in leaf (Morton) order.
+__global__ __launch_bounds__(256) void k_leaf_order(const TriP *__restrict__ tris, int ntri, int *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ntri) out[k] = tris[k].f.orig;
+}
+
 // Plain streaming copy (16 B per lane per step): the reference byte count for calibrating the
 // HBM traffic counters.
 __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
@@ -823,6 +829,19 @@ int ssdf_mesh_brute_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const float *x, 
     const unsigned nb = (unsigned)((npoints + 255) / 256);
     hipLaunchKernelGGL(k_brute_f64, dim3(nb), dim3(256), 0, (hipStream_t)stream, mesh->m.tris, (int)mesh->m.nsel,
                        x, y, z, (long long)npoints, out_d2, out_tri);
+    SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
+}
+
+int ssdf_mesh_leaf_order(ssdf_mesh_t mesh, int *out_tri, void *stream) {
+    if (!mesh || !out_tri) {
+        set_error("ssdf_mesh_leaf_order: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    if (mesh->m.nsel <= 0) return SSDF_OK;
+    const unsigned nb = (unsigned)((mesh->m.nsel + 255) / 256);
+    hipLaunchKernelGGL(k_leaf_order, dim3(nb), dim3(256), 0, (hipStream_t)stream, mesh->m.tris, (int)mesh->m.nsel,
+                       out_tri);
     SSDF_HIP_TRY(hipGetLastError());
     return SSDF_OK;
 }

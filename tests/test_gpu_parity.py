@@ -153,6 +153,21 @@ def test_lbvh_depth_matches_canonical_replay(oracle):
     assert info["open_edges"] == 0 and info["flipped_edges"] == 0
 
 
+@pytest.mark.parametrize("nu,nv", [(4, 3), (100, 40), (250, 200), (1000, 400)])
+def test_lbvh_leaf_order_is_the_stable_morton_sort(oracle, nu, nv):
+    """The build's radix sort against the oracle's stable sort by Morton code: 24 triangles (one
+    partial tile), 8 k, 100 k (tiles with the scan folded into hist and scatter) and 800 k (more
+    tiles than that: the separate row-scan kernel).  The pseudonormal pass sorts 3 T edge keys with
+    the same sort; a misordered edge sort splits shared edges, which the open-edge count shows."""
+    m = mg.torus(0.6, 0.2, nu, nv, sigma=0.005, seed=17)
+    rm = S.ResidentMesh.from_mesh(m, with_normals=True)
+    got = S.DeviceArray(m.s0.size, np.int32)
+    rm.leaf_order(got)
+    assert np.array_equal(got.download(), oracle.lbvh_order(m))
+    info = rm.info()
+    assert info["open_edges"] == 0 and info["flipped_edges"] == 0
+
+
 # ----------------------------------------------------------------------------------------------
 # noisy torus (config 2's mesh family) at test size, and the staircase sign fixture
 # ----------------------------------------------------------------------------------------------
