@@ -36,6 +36,10 @@ constexpr int kNone = INT_MIN;
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// v_writelane_b32: `old` with lane `lane` (wave-uniform) replaced by the wave-uniform `src`; one VALU
+// operation where a lane compare and a select take two, plus the VGPR copy of src.
+__device__ int amdgcn_writelane(int src, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 // largest float below x (x > 0), so that "y <= prev_f(x)" is "y < x"
 __device__ __forceinline__ float prev_f(float x) {
     if (x > 0.0f) return __int_as_float(__float_as_int(x) - 1);
@@ -172,12 +176,47 @@ __device__ __forceinline__ void packet_write(const QueryArgs &a, const long long
     if (a.out_tri) a.out_tri[qid] = tr.orig;
 }
 
+// One descent step's decision from the ballots of the node's two children (ml: lanes that want the
+// left child, mr: the right one, mle: lanes nearer the left box), all in scalar code:
+//   left first iff 2 |ml & (~mr | mle)| >= |ml | mr|: the majority of the wanting lanes is nearer to
+//     it, and a child that is alone in the vote wins it (left alone: 2|ml| >= |ml|; right alone:
+//     0 >= |mr| fails), so the taken child is wanted unless neither is;
+//   ent  = node | side << 31, the child not taken (side 1: right);
+//   sp  += 1 iff some lane wants the child not taken;
+//   node = the child taken, or kNone when no lane wants either.
+// Written out because the compiler keeps a uniform bool with several uses as a lane mask that each
+// use re-tests (29 scalar instructions per visit instead of these 15).
+__device__ __forceinline__ void descent_vote(const uint64_t ml, const uint64_t mr, const uint64_t mle, int &node,
+                                             const int left, const int right, int &sp, int &ent) {
+    uint64_t t, many;
+    int a, b, next;
+    asm("s_orn2_b64 %[t], %[mle], %[mr]\n\t"
+        "s_and_b64 %[t], %[t], %[ml]\n\t"
+        "s_bcnt1_i32_b64 %[a], %[t]\n\t"
+        "s_or_b64 %[many], %[ml], %[mr]\n\t"
+        "s_bcnt1_i32_b64 %[b], %[many]\n\t"
+        "s_lshl_b32 %[a], %[a], 1\n\t"
+        "s_cmp_ge_u32 %[a], %[b]\n\t"
+        "s_cselect_b32 %[next], %[left], %[right]\n\t"
+        "s_cselect_b64 %[t], %[mr], %[ml]\n\t"
+        "s_cselect_b32 %[a], 0x80000000, 0\n\t"
+        "s_or_b32 %[ent], %[node], %[a]\n\t"
+        "s_cmp_lg_u64 %[t], 0\n\t"
+        "s_addc_u32 %[sp], %[sp], 0\n\t"
+        "s_cmp_lg_u64 %[many], 0\n\t"
+        "s_cselect_b32 %[node], %[next], 0x80000000"
+        : [t] "=&s"(t), [many] "=&s"(many), [a] "=&s"(a), [b] "=&s"(b), [ent] "=&s"(ent), [next] "=&s"(next),
+          [sp] "+s"(sp), [node] "+s"(node)
+        : [ml] "s"(ml), [mr] "s"(mr), [mle] "s"(mle), [left] "s"(left), [right] "s"(right)
+        : "scc");
+}
+
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).
 //
 // Stack: wave-uniform ints, entry i in lane i % 64 of a VGPR, each naming where the pending subtree
-// hangs (parent << 1 | side); a pop re-reads that child's box from the parent record with scalar
-// loads and re-tests it.
+// hangs (parent | side << 31); a pop re-reads that child's box and reference from the parent record
+// with scalar loads and re-tests the box.
 // FIX candidates: per lane, an append-only list of leaf positions in LDS ([kCand][256], lane-major,
 // conflict-free).  A triangle enters when its FP32 d2 is inside band(best); a new best whose own
 // band lies below the old best empties the list first (every old entry is >= the old best, so
@@ -219,9 +258,9 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     float bestd = SSDF_INF;  // FIX: FP32 best distance found
 
 
-    // Stack of pending subtrees: entry i sits in lane i % 64 of VGPR i / 64 (written by a lane
-    // select at the wave-uniform depth sp, read back with v_readlane), each naming where the subtree
-    // hangs: parent << 1 | side.
+    // Stack of pending subtrees: entry i sits in lane i % 64 of VGPR i / 64 (written by v_writelane
+    // at the wave-uniform depth sp, read back with v_readlane), each naming where the subtree hangs:
+    // parent | side << 31.
     int stackv = 0, stackv_hi = 0;
     int sp = 0;
     bool overflow = false;
@@ -242,17 +281,17 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
             const uint64_t mr = __ballot(FIX ? dr <= thr : dr < best);
             const uint64_t mle = __ballot(dl <= dr);
-            const uint64_t many = ml | mr;
-            // the vote also picks a child that is alone in it: left alone wins 2|ml| >= |ml|,
-            // right alone loses 0 >= |mr|
-            const int lfirst = 2 * __popcll(ml & (~mr | mle)) >= __popcll(many) ? 1 : 0;
-            const int ent = (node << 1) | lfirst;  // the other child
-            // written at depth sp whether pushed or not: above the top it is overwritten before
-            // any pop reads it
-            stackv = lane == sp ? ent : stackv;
-            if (STACK == 128) stackv_hi = lane == sp - 64 ? ent : stackv_hi;
-            sp += (ml != 0 && mr != 0) ? 1 : 0;
-            node = uniform_i(many == 0 ? kNone : (lfirst ? nd.left : nd.right));
+            const int sp0 = sp;
+            int ent;
+            descent_vote(ml, mr, mle, node, nd.left, nd.right, sp, ent);
+            // the other child's entry is written at the old depth whether pushed or not (above the
+            // top it is overwritten before any pop reads it), by v_writelane from a scalar.  The
+            // depth is at most this node's depth < STACK: the stack holds at most one entry per
+            // level above the node, and the launch picks STACK from the tree depth.
+            if (STACK == 64 || sp0 < 64)
+                stackv = amdgcn_writelane(ent, sp0 & 63, stackv);
+            else
+                stackv_hi = amdgcn_writelane(ent, sp0 & 63, stackv_hi);
         }
         overflow |= sp > STACK;
         if (node != kNone) {
@@ -291,13 +330,15 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             --sp;
             const int e = (STACK == 64 || sp < 64) ? __builtin_amdgcn_readlane(stackv, sp & 63)
                                                    : __builtin_amdgcn_readlane(stackv_hi, sp & 63);
-            const float *pb = (const float *)(nodes + (e >> 1)) + 6 * (e & 1);
+            const int par = e & 0x7fffffff, side = (int)((unsigned)e >> 31);
+            const float *pb = (const float *)(nodes + par) + 6 * side;
             const float mn[3] = {pb[0], pb[1], pb[2]}, mx[3] = {pb[3], pb[4], pb[5]};
+            // the child reference is loaded with the box, not after the test passes (a second
+            // dependent scalar load)
+            node = uniform_i(((const int *)(nodes + par))[12 + side]);
             const float dd = box_d2_fast(qx, qy, qz, mn, mx);
-            if (__ballot(FIX ? dd <= thr : dd < best) != 0) {
-                node = uniform_i(((const int *)(nodes + (e >> 1)))[12 + (e & 1)]);
-                break;
-            }
+            if (__ballot(FIX ? dd <= thr : dd < best) != 0) break;
+            node = kNone;
         }
         if (node == kNone || overflow) break;
         if (++iters > lim) break;
