@@ -4,7 +4,8 @@ Every random draw comes from an in-repo SplitMix64 stream (Steele, Lea and Flood
 turned into 53-bit uniform doubles and, where needed, Box-Muller Gaussians, so a (generator, seed)
 pair names the same input on every machine.  Coordinates are returned as float32 SoA arrays and
 triangles as three int32 index arrays, the layout ssdf's C ABI takes (serial/ssdf.cpp:36-48).
-All meshes are closed and consistently oriented: counter-clockwise seen from outside.
+All meshes except deep_chain (a triangle soup for tree-depth tests) are closed and consistently
+oriented: counter-clockwise seen from outside.
 """
 from __future__ import annotations
 
@@ -290,6 +291,29 @@ def staircase(N: int = 4, H: float = 0.25) -> tuple[Mesh, list]:
                         hi = [max(c[0][dd], c[2][dd]) * H for dd in range(3)]
                         faces.append((axis, sign, lo, hi))
     return _mesh(np.array(verts, dtype=np.float64), np.array(tris), f"staircase_{N}"), faces
+
+
+def deep_chain(ndup: int = 8) -> Mesh:
+    """Open triangle soup whose LBVH is more than 64 levels deep (for the two-VGPR traversal stack).
+
+    Small triangles are centred at the origin (ndup copies), at (1, 1, 1), and at one point per
+    Morton bit: bit i of the 63-bit key is bit i // 3 of axis 2 - i % 3, so a centre at 2^(i // 3 - 21)
+    on that axis alone has the key 2^i once the centre bounds are [0, 1].  The sorted keys
+    0, 1, 2, 4, ..., 2^62 split one bit per level, and the ndup equal keys at 0 add log2(ndup) levels
+    of index splits below that.  Every coordinate is exact in float32, so are the box centres.
+    """
+    centres = [(0.0, 0.0, 0.0)] * ndup + [(1.0, 1.0, 1.0)]
+    for i in range(63):
+        c = [0.0, 0.0, 0.0]
+        c[2 - i % 3] = 2.0 ** (i // 3 - 21)
+        centres.append(tuple(c))
+    verts, tris = [], []
+    for c in centres:
+        d = max(c) * 2.0 ** -10 if max(c) > 0 else 2.0 ** -31
+        b = len(verts)
+        verts += [(c[0] - d, c[1] - d, c[2] - d), (c[0] + d, c[1] + d, c[2] + d), (c[0] + d, c[1] - d, c[2])]
+        tris.append((b, b + 1, b + 2))
+    return _mesh(np.array(verts, dtype=np.float64), np.array(tris), f"deep_chain_{ndup}")
 
 
 def padded_bbox(mesh: Mesh, pad: float = 0.1):

@@ -190,3 +190,25 @@ def test_destroy_after_the_stream_is_gone(meshes):
     rm2.query_grid(S.MODE_SDF, (8, 8, 8), (-1, -1, -1), (1, 1, 1), o2, None, flags=S.FLAG_FIXUP)
     assert np.all(np.isfinite(o2.download()))
     rm2.close()
+
+
+@pytest.mark.parametrize("flags", [S.FLAG_FIXUP, 0])
+def test_batch_with_a_tree_deeper_than_64(flags):
+    """A batch whose deepest tree passes 64 levels runs the two-VGPR-stack kernel for every mesh;
+    each result stays bit-identical to the single-mesh path."""
+    ms = [mg.deep_chain(8), mg.torus(0.6, 0.2, 60, 24, sigma=0.004, seed=5)]
+    dims, boxes = _grids(ms)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(ms)
+    assert 64 < b.info()["maxdepth"] <= 128
+    out = S.DeviceArray(total, np.float32)
+    tri = S.DeviceArray(total, np.int32)
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags)
+    q, err = b.last_stats()
+    assert err == 0
+    o, t = out.download(), tri.download()
+    for i, m in enumerate(ms):
+        so, st = _single_grid(m, dims[i], boxes[i], flags)
+        n = int(np.prod(dims[i]))
+        assert np.array_equal(o[off[i]:off[i] + n].view(np.int32), so.view(np.int32))
+        assert np.array_equal(t[off[i]:off[i] + n], st)

@@ -168,6 +168,43 @@ def test_lbvh_leaf_order_is_the_stable_morton_sort(oracle, nu, nv):
     assert info["open_edges"] == 0 and info["flipped_edges"] == 0
 
 
+def test_deep_lbvh_two_vgpr_stack(oracle):
+    """A tree deeper than 64 levels takes the traversal whose stack spans two VGPRs; every wave's
+    first descent (bound +inf) pushes at every level, so entries past 64 are written and popped.
+    Points mode (uniform, and clustered at the chain's deep end) and grid mode, FP32 + fixup and
+    plain, against the FP64 brute force."""
+    m = mg.deep_chain(8)
+    rm = S.ResidentMesh.from_mesh(m, with_normals=True)
+    assert 64 < rm.info()["maxdepth"] <= 128
+    diag = _diag(m)
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.uniform(-0.2, 1.2, (6000, 3)), rng.uniform(-1e-5, 1e-5, (3000, 3))]).astype(np.float32)
+    x, y, z = (np.ascontiguousarray(pts[:, i]) for i in range(3))
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    dx, dy, dz = _dev(x, y, z)
+    for flags in (S.FLAG_FIXUP, 0):
+        o = S.DeviceArray(x.size, np.float32)
+        t = S.DeviceArray(x.size, np.int32)
+        o.upload(np.full(x.size, np.inf, np.float32))
+        rm.query_points(S.MODE_EDF, dx, dy, dz, o, t, flags=flags)
+        q, err = rm.last_stats()
+        assert err == 0
+        got, gt = o.download(), t.download()
+        assert np.abs(got.astype(np.float64) - np.sqrt(d2)).max() <= 1e-6 * diag
+        if flags:
+            assert np.array_equal(gt, tri)
+    n = (21, 19, 23)
+    gx, gy, gz = mg.grid_points(*n, (-0.1, -0.1, -0.1), (1.1, 1.1, 1.1))
+    gd2, gtri, _, _ = oracle.brute_argmin(gx, gy, gz, m)
+    o = S.DeviceArray(gx.size, np.float32)
+    t = S.DeviceArray(gx.size, np.int32)
+    rm.query_grid(S.MODE_SDF, n, (-0.1, -0.1, -0.1), (1.1, 1.1, 1.1), o, t, flags=S.FLAG_FIXUP)
+    q, err = rm.last_stats()
+    assert err == 0
+    assert np.array_equal(t.download(), gtri)
+    assert np.abs(np.abs(o.download().astype(np.float64)) - np.sqrt(gd2)).max() <= 1e-6 * diag
+
+
 # ----------------------------------------------------------------------------------------------
 # noisy torus (config 2's mesh family) at test size, and the staircase sign fixture
 # ----------------------------------------------------------------------------------------------

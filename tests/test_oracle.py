@@ -408,6 +408,20 @@ def test_lbvh_replay_distances(oracle):
     assert nb.min() >= 2 and nt.min() >= 1
 
 
+@pytest.mark.parametrize("ndup,depth", [(1, 63), (2, 64), (8, 66), (64, 69)])
+def test_deep_chain_depth(oracle, ndup, depth):
+    """The deep-chain soup's keys 0, 2^0, ..., 2^62 split one bit per level (63 internal nodes on the
+    deepest path with one key at 0), and ndup equal keys at 0 add log2(ndup) index splits: the
+    GPU tests use ndup = 8 for a tree past the 64-entry stack."""
+    m = mg.deep_chain(ndup)
+    assert m.ntris == ndup + 64
+    x = np.array([0.3], np.float32)
+    _, _, md, d2 = oracle.lbvh_counts(x, x, x, m)
+    assert md == depth
+    bd2, _, _, _ = oracle.brute_argmin(x, x, x, m)
+    assert abs(math.sqrt(d2[0]) - math.sqrt(bd2[0])) < 1e-7
+
+
 # ----------------------------------------------------------------------------------------------
 # the driver's seeding helpers (serial/edf/edf.hpp:500-557)
 # ----------------------------------------------------------------------------------------------
