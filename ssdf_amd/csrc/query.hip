@@ -211,6 +211,20 @@ __device__ __forceinline__ void descent_vote(const uint64_t ml, const uint64_t m
         : "scc");
 }
 
+// Stack entry i (parent | side << 31): the pending child's box and reference, read from the parent
+// record with scalar loads issued together (the reference is not left for after the box test).
+template <int STACK>
+__device__ __forceinline__ void stack_entry_box(const Node *__restrict__ nodes, const int stackv, const int stackv_hi,
+                                                const int i, float *mn, float *mx, int &child) {
+    const int e = (STACK == 64 || i < 64) ? __builtin_amdgcn_readlane(stackv, i & 63)
+                                          : __builtin_amdgcn_readlane(stackv_hi, i & 63);
+    const int par = e & 0x7fffffff, side = (int)((unsigned)e >> 31);
+    const float *pb = (const float *)(nodes + par) + 6 * side;
+#pragma unroll
+    for (int d = 0; d < 3; d++) mn[d] = pb[d], mx[d] = pb[3 + d];
+    child = uniform_i(((const int *)(nodes + par))[12 + side]);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Wave-coherent FP32 traversal (float queries, float meshes).
 //
@@ -294,6 +308,12 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
                 stackv_hi = amdgcn_writelane(ent, sp0 & 63, stackv_hi);
         }
         overflow |= sp > STACK;
+        // The pop that follows re-reads the top entry's box and child reference; those scalar loads
+        // are issued here, before the leaf's, so that the two latencies overlap.  The box is tested
+        // after the leaf, against the bound the leaf may have tightened.
+        float pmn[3] = {0.f, 0.f, 0.f}, pmx[3] = {0.f, 0.f, 0.f};
+        int pchild = kNone;
+        if (sp > 0) stack_entry_box<STACK>(nodes, stackv, stackv_hi, sp - 1, pmn, pmx, pchild);
         if (node != kNone) {
             const int k = ~node;
             const TriP tr = tris[k];
@@ -324,21 +344,16 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
                 }
             }
         }
-        // pop the nearest pending subtree some lane still needs
+        // pop the nearest pending subtree some lane still needs (entry sp - 1 is in pmn/pmx/pchild)
         node = kNone;
         while (sp > 0) {
             --sp;
-            const int e = (STACK == 64 || sp < 64) ? __builtin_amdgcn_readlane(stackv, sp & 63)
-                                                   : __builtin_amdgcn_readlane(stackv_hi, sp & 63);
-            const int par = e & 0x7fffffff, side = (int)((unsigned)e >> 31);
-            const float *pb = (const float *)(nodes + par) + 6 * side;
-            const float mn[3] = {pb[0], pb[1], pb[2]}, mx[3] = {pb[3], pb[4], pb[5]};
-            // the child reference is loaded with the box, not after the test passes (a second
-            // dependent scalar load)
-            node = uniform_i(((const int *)(nodes + par))[12 + side]);
-            const float dd = box_d2_fast(qx, qy, qz, mn, mx);
-            if (__ballot(FIX ? dd <= thr : dd < best) != 0) break;
-            node = kNone;
+            const float dd = box_d2_fast(qx, qy, qz, pmn, pmx);
+            if (__ballot(FIX ? dd <= thr : dd < best) != 0) {
+                node = pchild;
+                break;
+            }
+            if (sp > 0) stack_entry_box<STACK>(nodes, stackv, stackv_hi, sp - 1, pmn, pmx, pchild);
         }
         if (node == kNone || overflow) break;
         if (++iters > lim) break;
