@@ -26,7 +26,7 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The HIP library; on a GPU box its absence is a failure, never a skip."""
+    """The HIP library; where a GPU test runs, its absence is a failure, never a skip."""
     import ssdf_amd
     L = ssdf_amd.lib()
     assert ssdf_amd.available(), "no GPU visible to libssdf_amd.so"
