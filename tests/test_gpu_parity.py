@@ -168,6 +168,20 @@ def test_lbvh_leaf_order_is_the_stable_morton_sort(oracle, nu, nv):
     assert info["open_edges"] == 0 and info["flipped_edges"] == 0
 
 
+@pytest.mark.parametrize("ntri", [1, 2, 4095, 4096, 4097, 786432, 786433])
+def test_lbvh_leaf_order_at_tile_boundaries(oracle, ntri):
+    """Random triangle soups of exactly these sizes around the sort's 4096-key tiles and its
+    192-tile limit for the folded scan (786,432 keys: folded; one more: the row-scan kernel)."""
+    rng = np.random.default_rng(ntri)
+    v = rng.uniform(-1.0, 1.0, (3 * ntri, 3)).astype(np.float32)
+    f = np.arange(3 * ntri, dtype=np.int32).reshape(ntri, 3)
+    m = mg.Mesh(*(np.ascontiguousarray(v[:, d]) for d in range(3)), *(np.ascontiguousarray(f[:, k]) for k in range(3)))
+    rm = S.ResidentMesh.from_mesh(m, with_normals=False)
+    got = S.DeviceArray(ntri, np.int32)
+    rm.leaf_order(got)
+    assert np.array_equal(got.download(), oracle.lbvh_order(m))
+
+
 def test_deep_lbvh_two_vgpr_stack(oracle):
     """A tree deeper than 64 levels takes the traversal whose stack spans two VGPRs; every wave's
     first descent (bound +inf) pushes at every level, so entries past 64 are written and popped.
