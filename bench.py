@@ -361,6 +361,20 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_valu_instructions(kernel_prefix):
+    """Wave-level VALU instructions per launch of the kernel (SQ_INSTS_VALU from the committed
+    rocprofv3 counter summary), or None."""
+    f = ROOT / "profiles" / "k_query_packet_torus200k_grid256_counters.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(kernel_prefix):
+            parts = line[len(kernel_prefix):].split()
+            if len(parts) == 8 and all(p.isdigit() for p in parts):  # the SQ table: SQ_INSTS_VALU is column 2
+                return int(parts[1])
+    return None
+
+
 def roofline(query_ms, n_queries, nverts, ntris):
     wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["c2_torus_200k_grid256"]
     assert wc["queries"] == n_queries
@@ -370,9 +384,14 @@ def roofline(query_ms, n_queries, nverts, ntris):
     # 12 B, index triples 12 B, FP32 BVH nodes 32 B x (2T - 1), FP32 triangle corners 36 B
     hbm_bytes = 8 * n_queries + 12 * nverts + 12 * ntris + 32 * (2 * ntris - 1) + 36 * ntris
     hbm = hbm_bytes / (query_ms * 1e-3) / 1e9
+    # the same launch against the instruction-issue limit: one wave-level VALU instruction per SIMD
+    # per 4 cycles, 1024 SIMDs at 2.4 GHz
+    valu = measured_valu_instructions("k_query_packet<64, 1, 1, true>")
+    issue = None if valu is None else round(valu * 4 / (1024 * 2.4e9 * query_ms * 1e-3), 4)
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic("k_query_packet_fixup"),
             "kernel": "k_query_packet (FP32+fixup)", "algorithmic_gflop_per_launch": round(flops / 1e9, 2),
+            "valu_issue_frac": issue,
             "hbm": {"achieved": round(hbm, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hbm / PEAK_HBM_GBS, 5),
                     "algorithmic_bytes_per_launch": hbm_bytes}}
 
