@@ -913,9 +913,10 @@ __device__ __forceinline__ void batch_args(const BatchArgs &b, const BatchMesh &
 // XCD the hardware dispatches it to, so each XCD's L2 holds mostly its range's meshes) and moves
 // on to the next ranges when its own is exhausted.
 template <int STACK, int QSRC, int OUT, bool FIX>
-// Occupancy, measured on the 64-mesh batch: plain runs best at 8 waves per SIMD (64 VGPRs), FIX
-// unconstrained (93 VGPRs, 5 waves): forced to 8 it spills 42 VGPRs and is 2% slower.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FIX ? 1 : 8, 8))) void k_batch_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
+// Occupancy, measured on the 64-mesh batch (FP32 + fixup, resident query): 8 waves per SIMD (64
+// VGPRs, the rest spilled outside the traversal loop) 76.5 ms, 7 waves 79.1 ms, 6 waves 83.2 ms,
+// unconstrained (92 VGPRs, 5 waves) 89.5 ms.  Plain runs best at 8 as well.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       const BatchMesh *__restrict__ meshes, BatchArgs b) {
     __shared__ int cand[FIX ? kCand : 1][256];
     const int lane = threadIdx.x & 63;
