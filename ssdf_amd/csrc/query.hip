@@ -37,7 +37,9 @@ constexpr int kNone = INT_MIN;
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // v_writelane_b32: `old` with lane `lane` (wave-uniform) replaced by the wave-uniform `src`; one VALU
-// operation where a lane compare and a select take two, plus the VGPR copy of src.
+// operation where a lane compare and a select take two, plus the VGPR copy of src.  The ROCm 7.2
+// clang has __builtin_amdgcn_readlane but no writelane builtin, so the declaration binds the LLVM
+// intrinsic by name.
 __device__ int amdgcn_writelane(int src, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
 // largest float below x (x > 0), so that "y <= prev_f(x)" is "y < x"
