@@ -455,9 +455,9 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
 }
 
 template <int STACK, int QSRC, int OUT, bool FIX>
-// 8 waves per SIMD (64 VGPRs): FIX spills 17 VGPRs and 30 SGPRs (none inside the descent loop; the
-// leaf and pop code reloads some) and still gains 8% on the 200 k-triangle torus at 256^3 over its
-// unconstrained 78 VGPRs and 6 waves.
+// 8 waves per SIMD (64 VGPRs): FIX spills 17 VGPRs and 32 SGPRs, all reloaded in the epilogue after
+// the walk (the FP64 candidate evaluation), none in the traversal loops, and still gains 8% on the
+// 200 k-triangle torus at 256^3 over its unconstrained 78 VGPRs and 6 waves.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
