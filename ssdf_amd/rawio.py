@@ -8,6 +8,7 @@ import numpy as np
 
 LIB = Path(__file__).resolve().parent / "_lib"
 SDF_EXE = LIB / "ssdf-sdf"
+MESH_TO_SDF_EXE = LIB / "ssdf-mesh_to_sdf"
 
 
 def write_surface(folder, m, ext: str = "typed") -> None:
@@ -30,3 +31,14 @@ def write_points(folder, x, y, z, ext: str = "typed") -> None:
 def read_field(path) -> np.ndarray:
     """float32[npoints] as written by the sdf command."""
     return np.fromfile(path, dtype=np.float32)
+
+
+def read_grid(folder):
+    """The mesh_to_sdf command's output: (field float32[nz, ny, nx], dims (nx, ny, nz), box
+    (xmin, ymin, zmin, xmax, ymax, zmax) as float32)."""
+    d = Path(folder)
+    meta = dict(line.split() for line in (d / "grid.txt").read_text().splitlines() if line.strip())
+    dims = tuple(int(meta[k]) for k in ("nx", "ny", "nz"))
+    box = tuple(np.float32(float(meta[k])) for k in ("xmin", "ymin", "zmin", "xmax", "ymax", "zmax"))
+    f = np.fromfile(d / "sdf.float32", dtype=np.float32)
+    return f.reshape(dims[2], dims[1], dims[0]), dims, box

@@ -1,5 +1,6 @@
-"""The sdf command (drivers/sdf.exe.cpp contract): argument and file checks that exit before any
-device work run on the CPU; the GPU runs are in test_gpu_cli.py."""
+"""The sdf and mesh_to_sdf commands (drivers/sdf.exe.cpp, demo/mesh_to_sdf.exe.cpp contracts):
+argument and file checks that exit before any device work run on the CPU; the GPU runs are in
+test_gpu_cli.py."""
 import subprocess
 
 import numpy as np
@@ -34,3 +35,29 @@ def test_missing_and_mismatched_files(tmp_path):
     np.zeros(4, np.float32).tofile(tmp_path / "p" / "z.float32")
     r = _run(tmp_path / "s", tmp_path / "p", tmp_path / "o.raw")
     assert r.returncode == 1 and "mismatched" in r.stderr
+
+
+def _run_m2s(*args, env=None):
+    if not rawio.MESH_TO_SDF_EXE.exists():
+        pytest.skip("ssdf-mesh_to_sdf not built")
+    return subprocess.run([str(rawio.MESH_TO_SDF_EXE), *map(str, args)], capture_output=True, text=True, env=env,
+                          timeout=120)
+
+
+def test_mesh_to_sdf_argument_checks(tmp_path):
+    import os
+    r = _run_m2s()
+    assert r.returncode == 1 and "Usage" in r.stderr
+    m = mg.uv_sphere(0.5, 8, 4)
+    rawio.write_surface(tmp_path / "s", m)
+    env = {k: v for k, v in os.environ.items() if not k.startswith("SSDF_")}
+    # grids need 2 nodes per axis unless SSDF_HMAX sets the spacing (mesh_to_sdf.exe.cpp:98-101)
+    r = _run_m2s(tmp_path / "s", 1, 5, 5, tmp_path / "o", env=env)
+    assert r.returncode == 1 and ">= 2" in r.stderr
+    r = _run_m2s(tmp_path / "s", "x", 5, 5, tmp_path / "o", env=env)
+    assert r.returncode == 1 and ">= 2" in r.stderr
+    r = _run_m2s(tmp_path / "nope", 5, 5, 5, tmp_path / "o", env=env)
+    assert r.returncode == 1 and "cannot read" in r.stderr
+    # block selection belongs to smesh meshes, which a raw folder is not
+    r = _run_m2s(tmp_path / "s", 5, 5, 5, tmp_path / "o", env=dict(env, SSDF_BOX_BLOCKS="a,b"))
+    assert r.returncode == 1 and "SSDF_BOX_BLOCKS" in r.stderr

@@ -75,3 +75,63 @@ def test_wrong_seed_size_fails(scene):
     r = subprocess.run([str(rawio.SDF_EXE), str(d / "a"), str(d / "p"), str(d / "o.raw")], capture_output=True,
                        text=True, env=e, timeout=120)
     assert r.returncode == 1 and "expected" in r.stderr
+
+
+def _expand_box(lo, hi, scale, margins):
+    """demo/mesh_to_sdf.exe.cpp:30-54 in float32: grow by the margins, scale about the centre."""
+    f = np.float32
+    out_lo, out_hi = [], []
+    for d in range(3):
+        c = (f(lo[d]) + f(hi[d])) * f(0.5)
+        h = ((f(hi[d]) - f(lo[d])) * f(0.5) + f(margins[d])) * f(scale)
+        out_lo.append(f(c - h))
+        out_hi.append(f(c + h))
+    return out_lo, out_hi
+
+
+def test_mesh_to_sdf(tmp_path, oracle):
+    import math
+    import ssdf_amd as S
+    m = mg.uv_sphere(0.5, 32, 16)
+    rawio.write_surface(tmp_path / "s", m)
+    lo = [m.sx.min(), m.sy.min(), m.sz.min()]
+    hi = [m.sx.max(), m.sy.max(), m.sz.max()]
+    env = {k: v for k, v in os.environ.items() if not k.startswith("SSDF_")}
+
+    def run(*args, **extra):
+        r = subprocess.run([str(rawio.MESH_TO_SDF_EXE), *map(str, args)], capture_output=True, text=True,
+                           env=dict(env, **{k: str(v) for k, v in extra.items()}), timeout=300)
+        assert r.returncode == 0, r.stderr
+        return rawio.read_grid(args[-1])
+
+    # margins (one per axis) and a box scale; distances negated
+    field, dims, box = run(tmp_path / "s", 17, 19, 21, tmp_path / "o1", SSDF_MARGIN="0.1", SSDF_MARGIN_Z="0.2",
+                           SSDF_SCALE_BOX="1.5", SSDF_SCALE="-1")
+    blo, bhi = _expand_box(lo, hi, 1.5, (0.1, 0.1, 0.2))
+    assert dims == (17, 19, 21)
+    assert [box[i] for i in range(3)] == blo and [box[3 + i] for i in range(3)] == bhi
+    x, y, z = mg.grid_points(17, 19, 21, blo, bhi)
+    s, _, _, _, _ = oracle.sdf(x, y, z, m)
+    diag = float(np.linalg.norm(np.array(bhi, np.float64) - np.array(blo, np.float64)))
+    got = -field.reshape(-1).astype(np.float64)
+    assert np.abs(np.abs(got) - np.abs(s)).max() <= 1e-6 * diag
+    far = np.abs(s) > 1e-6 * diag
+    assert np.array_equal(np.sign(got[far]), np.sign(s[far]))
+
+    # SSDF_HMAX: the node counts follow from the spacing; the field equals sample_sdf_bvh's
+    field, dims, box = run(tmp_path / "s", 0, 0, 0, tmp_path / "o2", SSDF_HMAX="0.07")
+    blo, bhi = _expand_box(lo, hi, 1.0, (0.0, 0.0, 0.0))
+    want_dims = tuple(int(math.ceil(float(np.float32(bhi[d] - blo[d])) / 0.07)) + 1 for d in range(3))
+    assert dims == want_dims
+    direct = np.empty(int(np.prod(dims)), np.float32)
+    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, *dims, *box[:3], *box[3:], direct, None, flags=0)
+    assert np.array_equal(field.reshape(-1), direct)
+
+    # SSDF_BOX_MESH: the box of another folder's points
+    pts = np.array([[-1.0, -0.5, -0.25], [1.0, 0.75, 0.5], [0.2, 0.1, 0.3]], np.float32)
+    rawio.write_points(tmp_path / "b", pts[:, 0], pts[:, 1], pts[:, 2])
+    field, dims, box = run(tmp_path / "s", 9, 8, 7, tmp_path / "o3", SSDF_BOX_MESH=tmp_path / "b")
+    assert box == tuple(np.float32(v) for v in (-1.0, -0.5, -0.25, 1.0, 0.75, 0.5))
+    direct = np.empty(9 * 8 * 7, np.float32)
+    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, 9, 8, 7, *box[:3], *box[3:], direct, None, flags=0)
+    assert np.array_equal(field.reshape(-1), direct)
