@@ -9,6 +9,8 @@ import numpy as np
 LIB = Path(__file__).resolve().parent / "_lib"
 SDF_EXE = LIB / "ssdf-sdf"
 MESH_TO_SDF_EXE = LIB / "ssdf-mesh_to_sdf"
+POINTS_INSIDE_EXE = LIB / "ssdf-points_inside"
+SDF_SIGN_EXE = LIB / "ssdf-sdf_sign"
 
 
 def write_surface(folder, m, ext: str = "typed") -> None:

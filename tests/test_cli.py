@@ -1,6 +1,7 @@
-"""The sdf and mesh_to_sdf commands (drivers/sdf.exe.cpp, demo/mesh_to_sdf.exe.cpp contracts):
-argument and file checks that exit before any device work run on the CPU; the GPU runs are in
-test_gpu_cli.py."""
+"""The sdf, mesh_to_sdf, points_inside and sdf_sign commands (drivers/sdf.exe.cpp,
+demo/mesh_to_sdf.exe.cpp, drivers/points_inside.exe.cpp, drivers/sdf_sign.exe.cpp contracts):
+argument and file checks that exit before any device work, and sdf_sign (a host file transform),
+run on the CPU; the GPU runs are in test_gpu_cli.py."""
 import subprocess
 
 import numpy as np
@@ -61,3 +62,42 @@ def test_mesh_to_sdf_argument_checks(tmp_path):
     # block selection belongs to smesh meshes, which a raw folder is not
     r = _run_m2s(tmp_path / "s", 5, 5, 5, tmp_path / "o", env=dict(env, SSDF_BOX_BLOCKS="a,b"))
     assert r.returncode == 1 and "SSDF_BOX_BLOCKS" in r.stderr
+
+
+def _run_exe(exe, *args):
+    if not exe.exists():
+        pytest.skip(f"{exe.name} not built")
+    return subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_sdf_sign(tmp_path):
+    """drivers/sdf_sign.exe.cpp:134-139: the distance negated where the inside flag is set."""
+    r = _run_exe(rawio.SDF_SIGN_EXE)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    rng = np.random.default_rng(5)
+    udf = rng.uniform(0.0, 2.0, 1001).astype(np.float32)
+    udf[:3] = (0.0, np.inf, 1e-30)
+    inside = (rng.uniform(size=1001) < 0.4).astype(np.uint8)
+    inside[:3] = (1, 1, 1)
+    udf.tofile(tmp_path / "u.raw")
+    inside.tofile(tmp_path / "in.raw")
+    r = _run_exe(rawio.SDF_SIGN_EXE, tmp_path / "u.raw", tmp_path / "in.raw", tmp_path / "s.raw")
+    assert r.returncode == 0, r.stderr
+    want = np.where(inside != 0, -udf, udf)
+    assert np.array_equal(np.fromfile(tmp_path / "s.raw", np.float32).view(np.uint32), want.view(np.uint32))
+    inside[:-1].tofile(tmp_path / "short.raw")
+    r = _run_exe(rawio.SDF_SIGN_EXE, tmp_path / "u.raw", tmp_path / "short.raw", tmp_path / "s2.raw")
+    assert r.returncode == 1 and "mismatch" in r.stderr
+
+
+def test_points_inside_argument_checks(tmp_path):
+    r = _run_exe(rawio.POINTS_INSIDE_EXE)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    m = mg.uv_sphere(0.5, 8, 4)
+    rawio.write_surface(tmp_path / "s", m, ext="raw")
+    rawio.write_points(tmp_path / "p", np.zeros(5), np.zeros(5), np.zeros(5), ext="raw")
+    r = _run_exe(rawio.POINTS_INSIDE_EXE, tmp_path / "nope", tmp_path / "p", tmp_path / "o.raw")
+    assert r.returncode == 1 and "Cannot open" in r.stderr
+    np.zeros(3, np.int32).tofile(tmp_path / "s" / "i1.raw")
+    r = _run_exe(rawio.POINTS_INSIDE_EXE, tmp_path / "s", tmp_path / "p", tmp_path / "o.raw")
+    assert r.returncode == 1 and "mismatched" in r.stderr

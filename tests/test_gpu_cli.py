@@ -135,3 +135,33 @@ def test_mesh_to_sdf(tmp_path, oracle):
     direct = np.empty(9 * 8 * 7, np.float32)
     S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, 9, 8, 7, *box[:3], *box[3:], direct, None, flags=0)
     assert np.array_equal(field.reshape(-1), direct)
+
+
+def test_points_inside_and_sdf_sign_chain(scene, oracle):
+    """drivers/points_inside.exe.cpp (surfaces concatenated, indices offset) and sdf_sign.exe.cpp
+    after the sdf command: the signed distance to the union, negative inside.  On closed surfaces
+    the exact winding number is 0 or 1, so every flag equals the oracle's (w >= 1/2)."""
+    d, (x, y, z), da, db = scene
+    a = mg.uv_sphere(0.5, 32, 16)
+    b = mg.torus(0.6, 0.2, 60, 24, sigma=0.003, seed=4)
+    b = mg.Mesh(b.sx + np.float32(1.5), b.sy, b.sz, b.s0, b.s1, b.s2)
+    nv = a.sx.size
+    union = mg.Mesh(np.concatenate([a.sx, b.sx]), np.concatenate([a.sy, b.sy]), np.concatenate([a.sz, b.sz]),
+                    np.concatenate([a.s0, b.s0 + nv]), np.concatenate([a.s1, b.s1 + nv]),
+                    np.concatenate([a.s2, b.s2 + nv]))
+    inside_want = (oracle.gwn(x, y, z, union) >= 0.5).astype(np.uint8)
+    assert 0 < inside_want.sum() < inside_want.size
+
+    def run(exe, *args):
+        r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+
+    run(rawio.SDF_EXE, d / "a", d / "b", d / "p", d / "u.raw")
+    run(rawio.POINTS_INSIDE_EXE, d / "a", d / "b_raw", d / "p", d / "in.raw")
+    inside = np.fromfile(d / "in.raw", np.uint8)
+    assert np.array_equal(inside, inside_want)
+    run(rawio.SDF_SIGN_EXE, d / "u.raw", d / "in.raw", d / "s.raw")
+    signed = np.fromfile(d / "s.raw", np.float32)
+    udf = np.minimum(da, db)
+    assert np.abs(np.abs(signed) - udf).max() <= 1e-6 * 4.0
+    assert np.array_equal(signed < 0, (inside_want == 1) & (udf > 0))
