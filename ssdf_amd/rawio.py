@@ -11,6 +11,7 @@ SDF_EXE = LIB / "ssdf-sdf"
 MESH_TO_SDF_EXE = LIB / "ssdf-mesh_to_sdf"
 POINTS_INSIDE_EXE = LIB / "ssdf-points_inside"
 SDF_SIGN_EXE = LIB / "ssdf-sdf_sign"
+SQEDF_EXE = LIB / "ssdf-sqedf"
 
 
 def write_surface(folder, m, ext: str = "typed") -> None:

@@ -101,3 +101,8 @@ def test_points_inside_argument_checks(tmp_path):
     np.zeros(3, np.int32).tofile(tmp_path / "s" / "i1.raw")
     r = _run_exe(rawio.POINTS_INSIDE_EXE, tmp_path / "s", tmp_path / "p", tmp_path / "o.raw")
     assert r.returncode == 1 and "mismatched" in r.stderr
+
+
+def test_sqedf_usage():
+    r = _run_exe(rawio.SQEDF_EXE)
+    assert r.returncode == 1 and "Usage" in r.stderr and "squared distance" in r.stderr

@@ -165,3 +165,37 @@ def test_points_inside_and_sdf_sign_chain(scene, oracle):
     udf = np.minimum(da, db)
     assert np.abs(np.abs(signed) - udf).max() <= 1e-6 * 4.0
     assert np.array_equal(signed < 0, (inside_want == 1) & (udf > 0))
+
+
+def test_sqedf_command(scene, oracle):
+    """drivers/sqedf.exe.cpp: squared distances to the union, min-accumulated per surface; FP64 mode
+    exact per surface and rounded to float (sqedf.exe.cpp:383-420); SSDF_INPUT_SDF holds squared
+    seeds."""
+    d, (x, y, z), da, db = scene
+
+    def run(*args, **env):
+        e = {k: v for k, v in os.environ.items() if not k.startswith("SSDF_")}
+        e.update({k: str(v) for k, v in env.items()})
+        r = subprocess.run([str(rawio.SQEDF_EXE), *map(str, args)], capture_output=True, text=True, env=e,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+
+    d2a, d2b = da ** 2, db ** 2
+    run(d / "a", d / "b_raw", d / "p", d / "q.raw")
+    got = rawio.read_field(d / "q.raw").astype(np.float64)
+    want = np.minimum(d2a, d2b)
+    assert np.abs(got - want).max() <= 2 * 1e-6 * 4.0 * np.sqrt(want.max())
+    # FP64: per surface, min(out, exact d^2) in double, then rounded to float
+    run(d / "a", d / "b", d / "p", d / "q64.raw", SSDF_DOUBLE_PRECISION=1)
+    f1 = np.minimum(np.float64(np.finfo(np.float32).max), oracle.brute_argmin(x, y, z, mg.uv_sphere(0.5, 32, 16))[0])
+    f1 = f1.astype(np.float32)
+    b = mg.torus(0.6, 0.2, 60, 24, sigma=0.003, seed=4)
+    b = mg.Mesh(b.sx + np.float32(1.5), b.sy, b.sz, b.s0, b.s1, b.s2)
+    f2 = np.minimum(f1.astype(np.float64), oracle.brute_argmin(x, y, z, b)[0]).astype(np.float32)
+    assert np.array_equal(rawio.read_field(d / "q64.raw"), f2)
+    # warm start from squared seeds
+    seed = np.full(x.size, 0.01, np.float32)
+    seed.tofile(d / "qseed.raw")
+    run(d / "a", d / "p", d / "qw.raw", SSDF_INPUT_SDF=d / "qseed.raw")
+    got = rawio.read_field(d / "qw.raw").astype(np.float64)
+    assert np.abs(got - np.minimum(np.float64(np.float32(0.01)), d2a)).max() <= 2 * 1e-6 * 4.0
