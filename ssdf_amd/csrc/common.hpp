@@ -19,8 +19,11 @@
 
 namespace ssdf_amd {
 
-// Internal LBVH node: both children's FP32 boxes, the child references (>= 0 internal node,
-// ~k leaf k) and the node's range of leaves [first, last] in Morton order.
+// Internal LBVH node: both children's FP32 boxes, the child references and the node's range of
+// leaves [first, last] in Morton order.  A child reference is >= 0 for an internal node, ~k for
+// leaf k, and ~(k | kPairLeaf) for the two leaves k and k + 1: a node whose children are both
+// single leaves is not visited, its parent's box test admits both triangles (leaf positions are
+// below 2^30, the build's triangle limit).
 struct alignas(16) Node {
     float lmin[3], lmax[3];
     float rmin[3], rmax[3];
@@ -28,6 +31,12 @@ struct alignas(16) Node {
     int first, last;
 };
 static_assert(sizeof(Node) == 64, "Node must be 64 bytes");
+
+constexpr int kPairLeaf = 1 << 30;
+
+// Leaf reference ref (< 0): its first leaf position, and 1 or 2 leaves.
+__host__ __device__ __forceinline__ int leaf_first(const int ref) { return ~ref & (kPairLeaf - 1); }
+__host__ __device__ __forceinline__ int leaf_count(const int ref) { return 1 + ((~ref >> 30) & 1); }
 
 // Far-field data of an internal node's two children for winding numbers: the child subtree's
 // summed area vector sum(a_i n_i), its area-weighted centroid and the squared radius of its box

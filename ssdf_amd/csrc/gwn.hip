@@ -156,7 +156,8 @@ __global__ __launch_bounds__(256) void k_gwn_nodes(const Node *__restrict__ node
         if (child >= 0) {
             f = nodes[child].first, l = nodes[child].last;
         } else {
-            f = l = ~child;
+            f = leaf_first(child);  // one leaf, or the two of a pair reference
+            l = f + leaf_count(child) - 1;
         }
         double S[7];
 #pragma unroll
@@ -228,8 +229,13 @@ __global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, cons
                 const float inv = 1.0f / (d2 * sqrtf(d2));
                 acc += (n[0] * dx + n[1] * dy + n[2] * dz) * inv;
             } else if (child < 0) {
-                const TriF t = tris[~child].f;
+                const int k0 = leaf_first(child);
+                const TriF t = tris[k0].f;
                 acc += solid_angle(qx, qy, qz, t.a, t.b, t.c);
+                if (leaf_count(child) > 1) {
+                    const TriF t1 = tris[k0 + 1].f;
+                    acc += solid_angle(qx, qy, qz, t1.a, t1.b, t1.c);
+                }
             } else if (sp < STACK) {
                 if (STACK == 64) {
                     stackv = lane == sp ? child : stackv;

@@ -390,6 +390,23 @@ __global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ 
         nd->rmin[d] = ordered_to_float(sl[6 + d]);
         nd->rmax[d] = ordered_to_float(sl[9 + d]);
     }
+    // a child whose own children are both single leaves becomes a pair reference.  Child c may be
+    // rewriting its own references meanwhile, but only ones that pointed at internal nodes, and
+    // neither the old nor the new value is a single leaf, so c is a pair either way or neither.
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const int c = side ? nd->right : nd->left;
+        if (c < 0) continue;
+        const int cl = nodes[c].left, cr = nodes[c].right;
+        const bool single_l = cl < 0 && !(~cl & kPairLeaf), single_r = cr < 0 && !(~cr & kPairLeaf);
+        if (single_l && single_r) {
+            const int ref = ~(~cl | kPairLeaf);  // leaves ~cl and ~cl + 1 (= ~cr)
+            if (side)
+                nd->right = ref;
+            else
+                nd->left = ref;
+        }
+    }
     if (i == 0) {
         meta->maxdepth = 1 + max(hts[0], hts[1]);  // leaves below the deepest path: internal nodes on it
         for (int d = 0; d < 3; d++) {

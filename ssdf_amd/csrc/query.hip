@@ -317,7 +317,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         int pchild = kNone;
         if (sp > 0) stack_entry_box<STACK>(nodes, stackv, stackv_hi, sp - 1, pmn, pmx, pchild);
         if (node != kNone) {
-            const int k = ~node;
+          // one leaf, or the two of a pair reference
+          const int k0 = leaf_first(node), nlv = leaf_count(node);
+          for (int j = 0; j < nlv; j++) {
+            const int k = k0 + j;
             const TriP tr = tris[k];
             const float d2 = tri_d2_fast(qx, qy, qz, tr);
             if (!FIX) {
@@ -345,6 +348,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
                     ncand = keep + 1;  // kCand + 1: more than kCand inside the band, exact path
                 }
             }
+          }
         }
         // pop the nearest pending subtree some lane still needs (entry sp - 1 is in pmn/pmx/pchild)
         node = kNone;
@@ -560,14 +564,18 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
                 continue;
             }
         } else {
-            const TriF tr = a.tris[~node].f;
-            double A[3], B[3], C[3];
-            tri_verts64<G>(a, tr, A, B, C);
-            const double d = pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
-            if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
-                best = d, bt = tr.orig, bk = ~node;
-                const double r = sqrt(best) + delta;
-                bound = r * r;
+            const int k0 = leaf_first(node), nlv = leaf_count(node);  // one leaf or a pair
+            for (int j = 0; j < nlv; j++) {
+                const TriF tr = a.tris[k0 + j].f;
+                double A[3], B[3], C[3];
+                tri_verts64<G>(a, tr, A, B, C);
+                const double d =
+                    pt_tri_d2<double>(qx, qy, qz, A[0], A[1], A[2], B[0], B[1], B[2], C[0], C[1], C[2]);
+                if (d < init2 && (d < best || (d == best && tr.orig < bt))) {
+                    best = d, bt = tr.orig, bk = k0 + j;
+                    const double r = sqrt(best) + delta;
+                    bound = r * r;
+                }
             }
         }
         node = kNone;
@@ -713,8 +721,17 @@ __device__ __forceinline__ bool coop_walk(const Node *__restrict__ nodes, const 
             p0 = box(nd.lmin, nd.lmax, bound);
             p1 = box(nd.rmin, nd.rmax, bound);
             c0 = nd.left, c1 = nd.right;
-            if (p0 && c0 < 0) imp |= leaf(~c0), p0 = false;
-            if (p1 && c1 < 0) imp |= leaf(~c1), p1 = false;
+            // leaf references: one leaf, or the two of a pair (both evaluated)
+            if (p0 && c0 < 0) {
+                imp |= leaf(leaf_first(c0));
+                if (leaf_count(c0) > 1) imp |= leaf(leaf_first(c0) + 1);
+                p0 = false;
+            }
+            if (p1 && c1 < 0) {
+                imp |= leaf(leaf_first(c1));
+                if (leaf_count(c1) > 1) imp |= leaf(leaf_first(c1) + 1);
+                p1 = false;
+            }
         }
         const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
         const int n = __popcll(m0) + __popcll(m1);
