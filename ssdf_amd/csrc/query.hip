@@ -5,8 +5,10 @@
 // record arrives with one scalar load, every lane tests its own query against both child boxes,
 // and two ballots decide whether to descend left, right or both (nearest-first by majority vote,
 // the far child pushed on a wave-uniform stack held one entry per lane in a VGPR).  Popped entries
-// are re-tested against each lane's current bound before the node is fetched again.  Leaves hold one triangle; the whole
-// wave evaluates it (point_triangle_dist_sq, serial/ssdf.hpp:50-121, in FP32).
+// are re-tested against each lane's current bound before the node is fetched again.  A leaf
+// reference names one triangle or a pair of consecutive leaves (the node above two single leaves is
+// never visited); the whole wave evaluates each triangle (point_triangle_dist_sq,
+// serial/ssdf.hpp:50-121, in FP32).
 //
 // FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
 // FP32 error bound of a distance, and each lane lists the triangles inside that band.  After
@@ -283,8 +285,9 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     int iters = 0;
     const int max_iters = (int)a.max_iters;
     int node = uniform_i(a.meta->root);
-    // The leaf budget shares the loop's one iteration test (lim): the hand-over below stays out of
-    // the loop body, where it cost the loop 5-10% even when never taken.
+    // The leaf budget (leaf references, a pair counting once) shares the loop's one iteration test
+    // (lim): the hand-over below stays out of the loop body, where it cost the loop 5-10% even when
+    // never taken.
     int lim = a.heavy_iters > 0 ? min(max_iters, a.heavy_iters) : max_iters;
     for (;;) {
     for (;;) {
