@@ -354,7 +354,7 @@ int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t
     }
     for (int i = 0; i < nmeshes; i++) {
         const ptrdiff_t t = tri_offsets[i + 1] - tri_offsets[i], v = vert_offsets[i + 1] - vert_offsets[i];
-        if (t < 0 || v < 0 || t > 0x3fffffff || (t > 0 && v == 0)) {
+        if (t < 0 || v < 0 || t >= ((ptrdiff_t)1 << kLeafBits) || (t > 0 && v == 0)) {
             set_error("ssdf_batch_create: offsets must not decrease; a mesh with triangles needs vertices");
             return SSDF_ERR_INVALID;
         }

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_gwn_nodes(const Node *__restrict__ node
         if (child >= 0) {
             f = nodes[child].first, l = nodes[child].last;
         } else {
-            f = leaf_first(child);  // one leaf, or the two of a pair reference
+            f = leaf_first(child);  // a run of leaf_count(child) leaves
             l = f + leaf_count(child) - 1;
         }
         double S[7];
@@ -229,12 +229,10 @@ __global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, cons
                 const float inv = 1.0f / (d2 * sqrtf(d2));
                 acc += (n[0] * dx + n[1] * dy + n[2] * dz) * inv;
             } else if (child < 0) {
-                const int k0 = leaf_first(child);
-                const TriF t = tris[k0].f;
-                acc += solid_angle(qx, qy, qz, t.a, t.b, t.c);
-                if (leaf_count(child) > 1) {
-                    const TriF t1 = tris[k0 + 1].f;
-                    acc += solid_angle(qx, qy, qz, t1.a, t1.b, t1.c);
+                const int k0 = leaf_first(child), nl = leaf_count(child);
+                for (int j = 0; j < nl; j++) {
+                    const TriF t = tris[k0 + j].f;
+                    acc += solid_angle(qx, qy, qz, t.a, t.b, t.c);
                 }
             } else if (sp < STACK) {
                 if (STACK == 64) {

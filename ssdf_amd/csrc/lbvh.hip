@@ -390,21 +390,18 @@ __global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ 
         nd->rmin[d] = ordered_to_float(sl[6 + d]);
         nd->rmax[d] = ordered_to_float(sl[9 + d]);
     }
-    // a child whose own children are both single leaves becomes a pair reference.  Child c may be
-    // rewriting its own references meanwhile, but only ones that pointed at internal nodes, and
-    // neither the old nor the new value is a single leaf, so c is a pair either way or neither.
+    // a child subtree of at most kMaxRun leaves becomes a run reference (its range is contiguous in
+    // Morton order and fixed by k_karras, so no other thread of this kernel changes it)
 #pragma unroll
     for (int side = 0; side < 2; side++) {
         const int c = side ? nd->right : nd->left;
         if (c < 0) continue;
-        const int cl = nodes[c].left, cr = nodes[c].right;
-        const bool single_l = cl < 0 && !(~cl & kPairLeaf), single_r = cr < 0 && !(~cr & kPairLeaf);
-        if (single_l && single_r) {
-            const int ref = ~(~cl | kPairLeaf);  // leaves ~cl and ~cl + 1 (= ~cr)
+        const int f = nodes[c].first, n = nodes[c].last - f + 1;
+        if (n <= kMaxRun) {
             if (side)
-                nd->right = ref;
+                nd->right = leaf_run(f, n);
             else
-                nd->left = ref;
+                nd->left = leaf_run(f, n);
         }
     }
     if (i == 0) {
@@ -420,7 +417,7 @@ static unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b);
 
 int build_lbvh(DevMesh &m, hipStream_t stream) {
     const int n = (int)m.nsel;
-    if (m.nsel <= 0 || m.nsel > 0x3fffffff) {
+    if (m.nsel <= 0 || m.nsel >= ((ptrdiff_t)1 << kLeafBits)) {
         set_error("build_lbvh: triangle count out of range");
         return SSDF_ERR_INVALID;
     }

@@ -6,8 +6,8 @@
 // and two ballots decide whether to descend left, right or both (nearest-first by majority vote,
 // the far child pushed on a wave-uniform stack held one entry per lane in a VGPR).  Popped entries
 // are re-tested against each lane's current bound before the node is fetched again.  A leaf
-// reference names one triangle or a pair of consecutive leaves (the node above two single leaves is
-// never visited); the whole wave evaluates each triangle (point_triangle_dist_sq,
+// reference names a run of consecutive leaves (subtrees of at most kMaxRun leaves are never
+// visited); the whole wave evaluates each triangle (point_triangle_dist_sq,
 // serial/ssdf.hpp:50-121, in FP32).
 //
 // FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
@@ -285,7 +285,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     int iters = 0;
     const int max_iters = (int)a.max_iters;
     int node = uniform_i(a.meta->root);
-    // The leaf budget (leaf references, a pair counting once) shares the loop's one iteration test
+    // The leaf budget (leaf references, a run counting once) shares the loop's one iteration test
     // (lim): the hand-over below stays out of the loop body, where it cost the loop 5-10% even when
     // never taken.
     int lim = a.heavy_iters > 0 ? min(max_iters, a.heavy_iters) : max_iters;
@@ -320,7 +320,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         int pchild = kNone;
         if (sp > 0) stack_entry_box<STACK>(nodes, stackv, stackv_hi, sp - 1, pmn, pmx, pchild);
         if (node != kNone) {
-          // one leaf, or the two of a pair reference
+          // a run of leaves (one, or the leaves of a subtree too small to visit)
           const int k0 = leaf_first(node), nlv = leaf_count(node);
           for (int j = 0; j < nlv; j++) {
             const int k = k0 + j;
@@ -567,7 +567,7 @@ __device__ void exact_one(const QueryArgs &a, const long long qid, const double 
                 continue;
             }
         } else {
-            const int k0 = leaf_first(node), nlv = leaf_count(node);  // one leaf or a pair
+            const int k0 = leaf_first(node), nlv = leaf_count(node);  // a run of leaves
             for (int j = 0; j < nlv; j++) {
                 const TriF tr = a.tris[k0 + j].f;
                 double A[3], B[3], C[3];
@@ -724,15 +724,13 @@ __device__ __forceinline__ bool coop_walk(const Node *__restrict__ nodes, const 
             p0 = box(nd.lmin, nd.lmax, bound);
             p1 = box(nd.rmin, nd.rmax, bound);
             c0 = nd.left, c1 = nd.right;
-            // leaf references: one leaf, or the two of a pair (both evaluated)
+            // leaf references: a run of leaves, every one evaluated
             if (p0 && c0 < 0) {
-                imp |= leaf(leaf_first(c0));
-                if (leaf_count(c0) > 1) imp |= leaf(leaf_first(c0) + 1);
+                for (int j = 0; j < leaf_count(c0); j++) imp |= leaf(leaf_first(c0) + j);
                 p0 = false;
             }
             if (p1 && c1 < 0) {
-                imp |= leaf(leaf_first(c1));
-                if (leaf_count(c1) > 1) imp |= leaf(leaf_first(c1) + 1);
+                for (int j = 0; j < leaf_count(c1); j++) imp |= leaf(leaf_first(c1) + j);
                 p1 = false;
             }
         }
