@@ -173,7 +173,10 @@ typedef struct {
 } ssdf_mesh_info_t;
 
 /* dtype 0: float vertices, 1: double.  device_ptrs != 0: the arrays are device memory and are
- * copied on the device.  with_normals != 0: also build the FP64 pseudonormals (needed by SDF). */
+ * copied on the device.  with_normals != 0: also build the FP64 pseudonormals (needed by SDF).
+ * A mesh holds at most 2^29 - 1 triangles (SSDF_ERR_INVALID beyond): the BVH's leaf references
+ * carry a leaf position and a run length in one int.  The same limit holds for every entry point
+ * that builds a BVH, and per mesh in batches. */
 int ssdf_mesh_create(int dtype, ptrdiff_t nselements, const int *s0, const int *s1, const int *s2,
                      ptrdiff_t nspoints, const void *sx, const void *sy, const void *sz, int device_ptrs,
                      int with_normals, void *stream, ssdf_mesh_t *out);
