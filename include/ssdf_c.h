@@ -267,11 +267,12 @@ int ssdf_batch_info(ssdf_batch_t batch, int mesh, ssdf_mesh_info_t *info);
 
 /* ---------------------------------------------------------------------------------------------
  * Serving (no reference counterpart).  A worker thread keeps registered meshes resident (built on
- * first use, least recently used idle meshes evicted when the HBM budget needs room), admits
- * requests in arrival order when their device buffers (20 B per query) fit the budget, and runs
- * them as soon as they are admitted on a pool of streams (upload, query, download).  Requests
- * take HOST arrays that must stay valid until the request completes; pinned memory
- * (ssdf_host_alloc) lets the copies run asynchronously.
+ * first use on a stream of their own, least recently used idle meshes evicted when the HBM budget
+ * needs room), admits requests in arrival order when their device buffer (20 B per query of
+ * inputs and outputs plus the request's own query workspace) fits the budget, and runs them as soon
+ * as they are admitted on a pool of streams (upload, query, download), requests on the same mesh
+ * included.  Requests take HOST arrays that must stay valid until the request completes; pinned
+ * memory (ssdf_host_alloc) lets the copies run asynchronously.
  * --------------------------------------------------------------------------------------------- */
 typedef struct ssdf_server *ssdf_server_t;
 
@@ -287,6 +288,11 @@ int ssdf_server_create(size_t hbm_budget_bytes, int max_inflight, ssdf_server_t 
 /* Registers an FP32 mesh (host arrays kept by the caller for the server's lifetime). */
 int ssdf_server_add_mesh(ssdf_server_t server, int mesh_id, ptrdiff_t nselements, const int *s0, const int *s1,
                          const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy, const float *sz);
+/* The HBM the server charges against its budget for mesh mesh_id while it is resident (an upper
+ * bound of what its tree, normals, winding-number tree and build workspace can allocate) and for a
+ * request of npoints points (its inputs, outputs and query workspace); either pointer may be NULL. */
+int ssdf_server_charge(ssdf_server_t server, int mesh_id, ptrdiff_t npoints, size_t *mesh_bytes,
+                       size_t *request_bytes);
 /* Seconds on the server's clock (its creation is 0). */
 int ssdf_server_clock(ssdf_server_t server, double *now);
 /* mode EDF (out_d and out_tri read as the current best, kept where not improved) or SDF (both

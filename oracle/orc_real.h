@@ -429,7 +429,7 @@ int FN(orc_celllist)(const ptrdiff_t npoints, const REAL *x, const REAL *y, cons
     {
         FN(orc_cellrec) *tmp = NULL;
         ptrdiff_t cap = 0;
-#pragma omp for schedule(dynamic, 64)
+#pragma omp for /* the reference's default (static) schedule, ssdf.hpp:819 */
         for (ptrdiff_t c = 0; c < ncells; ++c) {
             const ptrdiff_t b = cell_ptr[c], e = cell_ptr[c + 1];
             if (e - b < 2) continue;
@@ -472,8 +472,9 @@ int FN(orc_celllist)(const ptrdiff_t npoints, const REAL *x, const REAL *y, cons
     const REAL *tminx = tmin[0], *tminy = tmin[1], *tminz = tmin[2];
     const REAL *tmaxx = tmax[0], *tmaxy = tmax[1], *tmaxz = tmax[2];
 
-    /* queries (912-959): own cell first, then every other non-empty cell that can improve */
-#pragma omp parallel for schedule(dynamic, 256)
+    /* queries (912-959): own cell first, then every other non-empty cell that can improve; the
+     * reference's default (static) schedule of its `omp parallel for` (912) */
+#pragma omp parallel for
     for (ptrdiff_t p = 0; p < npoints; ++p) {
         const REAL px = x[p], py = y[p], pz = z[p];
         const REAL pcoord = xyz[sort_axis][p];

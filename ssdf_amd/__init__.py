@@ -107,6 +107,7 @@ _SIGS = {
     "ssdf_batch_info": [_P, C.c_int, C.POINTER(_MeshInfo)],
     "ssdf_server_create": [C.c_size_t, C.c_int, C.POINTER(_P)],
     "ssdf_server_add_mesh": [_P, C.c_int, _PD, _P, _P, _P, _PD, _P, _P, _P],
+    "ssdf_server_charge": [_P, C.c_int, _PD, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "ssdf_server_clock": [_P, C.POINTER(C.c_double)],
     "ssdf_server_submit": [_P, C.c_int, C.c_int, _PD, _P, _P, _P, _P, _P, C.c_uint, C.c_double,
                            C.POINTER(C.c_uint64)],
@@ -677,6 +678,13 @@ class Server:
         rc = lib().ssdf_server_add_mesh(C.c_void_p(self.h), mesh_id, arrs[0].size, *[_ptr(a) for a in arrs[:3]],
                                         arrs[3].size, *[_ptr(a) for a in arrs[3:]])
         _check(rc, "ssdf_server_add_mesh")
+
+    def charge(self, mesh_id: int, npoints: int) -> tuple[int, int]:
+        """(bytes charged for the resident mesh, bytes charged for a request of npoints points)."""
+        mb, rb = C.c_size_t(), C.c_size_t()
+        _check(lib().ssdf_server_charge(C.c_void_p(self.h), mesh_id, int(npoints), C.byref(mb), C.byref(rb)),
+               "ssdf_server_charge")
+        return int(mb.value), int(rb.value)
 
     def clock(self) -> float:
         t = C.c_double()

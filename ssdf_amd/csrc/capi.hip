@@ -17,10 +17,11 @@ namespace {
 // FP32 error bound of a computed distance, relative to the largest coordinate magnitude involved:
 // 32 ulps of 1.0 (a few roundings of the projection and of p - q, with ample margin).
 
+// Query workspace a handle keeps between calls (grown on demand, stream-ordered).
 struct Workspace {
     unsigned *status = nullptr;  // [0] queue count, [1] kernel error flag, [2] queue read position
     unsigned *list = nullptr;
-    size_t cap = 0;
+    size_t cap = 0;  // words of list (4 per queue entry)
     void *sort = nullptr;  // Morton sort of scattered query points
     size_t sort_cap = 0;
 };
@@ -126,10 +127,6 @@ int upload_mesh(DevMesh &m, ptrdiff_t nsel, const int *s0, const int *s1, const 
     return SSDF_OK;
 }
 
-float mesh_scale(const DevMesh &m) { return meta_scale(m.hmeta); }
-
-float mesh_diag(const DevMesh &m) { return meta_diag(m.hmeta); }
-
 }  // namespace
 
 struct ssdf_mesh {
@@ -137,17 +134,8 @@ struct ssdf_mesh {
     Workspace ws;
     hipStream_t last_stream = nullptr;  // stream of the last query (its workspace use)
     bool queried = false;
+    hipEvent_t order_ev = nullptr;  // orders one stream after another (stream_after)
 };
-
-namespace {
-
-// Queue entries reserved for nq queries (FIX near-ties that overflow the candidate list, and the
-// queries of packet waves past their leaf budget): an eighth of the queries, at least 64 Ki.  Past
-// that a near-tie keeps its FP32 result and sets bit 4 of the kernel error flag, and a heavy query
-// stays in its packet wave.
-size_t fixup_entries(size_t nq) { return std::min(nq, std::max<size_t>(1u << 16, nq / 8)); }
-
-}  // namespace
 
 namespace {
 std::atomic<int> &heavy_setting() {
@@ -165,30 +153,59 @@ int ssdf_amd::heavy_leaves() { return heavy_setting().load(std::memory_order_rel
 
 namespace {
 
-int prepare_args(ssdf_mesh *h, QueryArgs &a, size_t nq, unsigned flags, hipStream_t stream) {
-    if (h->queried && h->last_stream != stream && h->ws.list) {
-        // the workspace was last used on another stream: order this use after it
-        hipEvent_t ev;
-        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        SSDF_HIP_TRY(hipEventRecord(ev, h->last_stream));
-        SSDF_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
-        (void)hipEventDestroy(ev);
-    }
+// Makes `st` wait for the work queued so far on `other` (nothing when they are one stream).
+int stream_after(ssdf_mesh *h, hipStream_t st, hipStream_t other) {
+    if (st == other) return SSDF_OK;
+    if (!h->order_ev) SSDF_HIP_TRY(hipEventCreateWithFlags(&h->order_ev, hipEventDisableTiming));
+    SSDF_HIP_TRY(hipEventRecord(h->order_ev, other));
+    SSDF_HIP_TRY(hipStreamWaitEvent(st, h->order_ev, 0));
+    return SSDF_OK;
+}
+
+// Orders `st` after every use of the handle's device buffers so far: its builds (the handle's
+// stream) and its last query (whose kernels read the tree and the triangles).
+int order_after_handle(ssdf_mesh *h, hipStream_t st) {
+    if (int rc = stream_after(h, st, h->m.stream)) return rc;
+    if (h->queried)
+        if (int rc = stream_after(h, st, h->last_stream)) return rc;
+    return SSDF_OK;
+}
+
+// Fills the mesh-independent launch arguments of a query over nq queries working in `w`, and clears
+// w's status words (a kernel, so that the sequence stays capturable).
+void fill_args(const DevMesh &m, QueryArgs &a, const QueryWs &w, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, w.status);
+    a.fix_count = w.status;
+    a.err_flag = (int *)(w.status + 1);
+    a.fix_list = w.list;
+    a.fix_cap = (unsigned)std::min<size_t>(w.entries, 0xffffffffu);
+    a.heavy_iters = heavy_leaves();
+    a.heavy_cap = a.fix_cap / 2;
+    const float L = meta_scale(m.hmeta);
+    a.tau = kTau;
+    a.band_abs = kBandRel * L;
+    a.sign_eps = (float)(kSignEpsRel * meta_diag(m.hmeta));
+    a.delta64 = kDelta64Rel * L;
+}
+
+// The handle's own workspace for a query of nq queries, ordered after its last use on another
+// stream (calls on one handle share it).
+int handle_ws(ssdf_mesh *h, size_t nq, bool sorted, hipStream_t stream, QueryWs &w) {
+    // after the handle's last build (it may have run on another stream) ...
+    if (int rc = stream_after(h, stream, h->m.stream)) return rc;
+    // ... and after the last query that used this workspace
+    if (h->queried && h->last_stream != stream && h->ws.list)
+        if (int rc = stream_after(h, stream, h->last_stream)) return rc;
     h->queried = true;
     h->last_stream = stream;
     if (int rc = ws_reserve(h->ws, 4 * fixup_entries(nq), stream)) return rc;
-    hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, h->ws.status);
-    a.fix_count = h->ws.status;
-    a.err_flag = (int *)(h->ws.status + 1);
-    a.fix_list = h->ws.list;
-    a.fix_cap = (unsigned)std::min<size_t>(h->ws.cap / 4, 0xffffffffu);
-    a.heavy_iters = heavy_leaves();
-    a.heavy_cap = a.fix_cap / 2;
-    const float L = mesh_scale(h->m);
-    a.tau = kTau;
-    a.band_abs = kBandRel * L;
-    a.sign_eps = (float)(kSignEpsRel * mesh_diag(h->m));
-    a.delta64 = kDelta64Rel * L;
+    if (sorted)
+        if (int rc = ws_sort_reserve(h->ws, sort_ws_bytes((long long)nq), stream)) return rc;
+    w.status = h->ws.status;
+    w.list = h->ws.list;
+    w.entries = h->ws.cap / 4;
+    w.sort = sorted ? h->ws.sort : nullptr;
+    w.sort_bytes = sorted ? h->ws.sort_cap : 0;
     return SSDF_OK;
 }
 
@@ -226,21 +243,15 @@ void destroy_mesh(ssdf_mesh *h, bool streams_alive) {
     if (!h) return;
     hipStream_t st = nullptr;
     if (streams_alive) {
+        // frees are ordered on the build stream: make it wait for the last query's stream
         st = h->m.stream;
-        if (h->queried && h->last_stream != st) {
-            // frees are ordered on the creation stream: make it wait for the last query's stream
-            hipEvent_t ev;
-            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-                (void)hipEventRecord(ev, h->last_stream);
-                (void)hipStreamWaitEvent(st, ev, 0);
-                (void)hipEventDestroy(ev);
-            }
-        }
+        if (h->queried) (void)stream_after(h, st, h->last_stream);
     } else {
         (void)hipDeviceSynchronize();
     }
     ws_free(h->ws, st);
     free_mesh(h->m, st);
+    if (h->order_ev) (void)hipEventDestroy(h->order_ev);
     delete h;
 }
 
@@ -285,31 +296,46 @@ int fill_empty(int mode, size_t n, float *out_w, unsigned char *out_in, hipStrea
     return SSDF_OK;
 }
 
-int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y, const void *z, void *out_d,
-                 int *out_tri, void *cx, void *cy, void *cz, unsigned flags, hipStream_t stream) {
-    if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
-    if (!x || !y || !z || !out_d || (mode == SSDF_MODE_CLOSEST && (!cx || !cy || !cz))) {
-        set_error("query_points: null argument");
-        return SSDF_ERR_INVALID;
-    }
+bool points_sorted(const DevMesh &m, ptrdiff_t n) { return m.dtype == 0 && n >= kSortMin; }
+
+// A point query of mode `mode` on mesh m, working in w (which needs the sort part when
+// points_sorted(m, n)).
+int query_points_in(DevMesh &m, const QueryWs &w, int mode, ptrdiff_t n, const void *x, const void *y,
+                    const void *z, void *out_d, int *out_tri, void *cx, void *cy, void *cz, unsigned flags,
+                    hipStream_t stream) {
     QueryArgs a{};
-    if (int rc = prepare_args(h, a, (size_t)n, flags, stream)) return rc;
+    fill_args(m, a, w, stream);
     a.qx = x, a.qy = y, a.qz = z;
     a.npoints = n;
     a.out_d = out_d;
     a.out_tri = out_tri;
     a.out_cx = cx, a.out_cy = cy, a.out_cz = cz;
-    const int dbl = h->m.dtype;
-    if (dbl == 0 && n >= kSortMin) {
-        if (int rc = ws_sort_reserve(h->ws, sort_ws_bytes(n), stream)) return rc;
-        if (int rc = sort_points(h->m.meta, n, (const float *)x, (const float *)y, (const float *)z, h->ws.sort,
-                                 &a.perm, stream))
+    const int dbl = m.dtype;
+    if (points_sorted(m, n))
+        if (int rc = sort_points(m.meta, n, (const float *)x, (const float *)y, (const float *)z, w.sort, &a.perm,
+                                 stream))
             return rc;
-    }
-    if (int rc = launch_query(h->m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream)) return rc;
-    if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags))
-        return launch_gwn(h->m, a, QSRC_POINTS, GW_SIGN, dbl, stream);
+    if (int rc = launch_query(m, a, QSRC_POINTS, mode, flags, dbl, dbl, stream)) return rc;
+    if (mode == SSDF_MODE_SDF && gwn_sign_wanted(m, flags)) return launch_gwn(m, a, QSRC_POINTS, GW_SIGN, dbl, stream);
     return SSDF_OK;
+}
+
+int check_points_args(int mode, const void *x, const void *y, const void *z, const void *out_d, const void *cx,
+                      const void *cy, const void *cz) {
+    if (!x || !y || !z || !out_d || (mode == SSDF_MODE_CLOSEST && (!cx || !cy || !cz))) {
+        set_error("query_points: null argument");
+        return SSDF_ERR_INVALID;
+    }
+    return SSDF_OK;
+}
+
+int query_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y, const void *z, void *out_d,
+                 int *out_tri, void *cx, void *cy, void *cz, unsigned flags, hipStream_t stream) {
+    if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
+    if (int rc = check_points_args(mode, x, y, z, out_d, cx, cy, cz)) return rc;
+    QueryWs w;
+    if (int rc = handle_ws(h, (size_t)n, points_sorted(h->m, n), stream, w)) return rc;
+    return query_points_in(h->m, w, mode, n, x, y, z, out_d, out_tri, cx, cy, cz, flags, stream);
 }
 
 int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, const double *lo, const double *hi,
@@ -319,9 +345,11 @@ int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
         set_error("query_grid: invalid grid");
         return SSDF_ERR_INVALID;
     }
-    QueryArgs a{};
     const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
-    if (int rc = prepare_args(h, a, nq, flags, stream)) return rc;
+    QueryWs w;
+    if (int rc = handle_ws(h, nq, false, stream, w)) return rc;
+    QueryArgs a{};
+    fill_args(h->m, a, w, stream);
     setup_grid(h->m, a, nx, ny, nz, lo, hi);
     a.out_d = out_d;
     a.out_tri = out_tri;
@@ -341,18 +369,18 @@ int gwn_points(ssdf_mesh *h, int mode, ptrdiff_t n, const void *x, const void *y
         return SSDF_ERR_INVALID;
     }
     if (h->m.nsel <= 0) return fill_empty(mode, (size_t)n, out_w, out_in, stream);
+    QueryWs w;
+    if (int rc = handle_ws(h, (size_t)n, points_sorted(h->m, n), stream, w)) return rc;
     QueryArgs a{};
-    if (int rc = prepare_args(h, a, (size_t)n, 0, stream)) return rc;
+    fill_args(h->m, a, w, stream);
     a.qx = x, a.qy = y, a.qz = z;
     a.npoints = n;
     a.out_w = out_w;
     a.out_in = out_in;
-    if (h->m.dtype == 0 && n >= kSortMin) {
-        if (int rc = ws_sort_reserve(h->ws, sort_ws_bytes(n), stream)) return rc;
-        if (int rc = sort_points(h->m.meta, n, (const float *)x, (const float *)y, (const float *)z, h->ws.sort,
-                                 &a.perm, stream))
+    if (points_sorted(h->m, n))
+        if (int rc = sort_points(h->m.meta, n, (const float *)x, (const float *)y, (const float *)z, w.sort, &a.perm,
+                                 stream))
             return rc;
-    }
     return launch_gwn(h->m, a, QSRC_POINTS, mode, h->m.dtype, stream);
 }
 
@@ -365,8 +393,10 @@ int gwn_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, c
     }
     const size_t nq = (size_t)nx * (size_t)ny * (size_t)nz;
     if (h->m.nsel <= 0) return fill_empty(mode, nq, out_w, out_in, stream);
+    QueryWs w;
+    if (int rc = handle_ws(h, nq, false, stream, w)) return rc;
     QueryArgs a{};
-    if (int rc = prepare_args(h, a, nq, 0, stream)) return rc;
+    fill_args(h->m, a, w, stream);
     setup_grid(h->m, a, nx, ny, nz, lo, hi);
     a.out_w = out_w;
     a.out_in = out_in;
@@ -582,6 +612,30 @@ int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t ns
     *bytes = h->m.device_bytes + h->m.ws_cap;
     return SSDF_OK;
 }
+
+// A point query in a caller-provided contiguous workspace `ws` (256-byte aligned,
+// query_ws_bytes(n, n >= kSortMin) bytes) instead of the handle's: calls on one mesh then share
+// only its read-only tree and may run concurrently on different streams.  The winding-number tree
+// that SSDF_FLAG_SIGN_GWN / AUTO need is built here (allocating, synchronous) when may_build_gwn,
+// else its absence is an error.
+int ws_query_points(ssdf_mesh_t h, void *ws, bool may_build_gwn, int mode, ptrdiff_t n, const void *x,
+                    const void *y, const void *z, void *out_d, int *out_tri, void *cx, void *cy, void *cz,
+                    unsigned flags, hipStream_t stream) {
+    if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
+    if (int rc = check_points_args(mode, x, y, z, out_d, cx, cy, cz)) return rc;
+    if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags) && !h->m.has_gwn) {
+        if (!may_build_gwn) {
+            set_error("the mesh's winding-number tree is not built (run one allocating winding-number or "
+                      "signed query on it first)");
+            return SSDF_ERR_INVALID;
+        }
+        // built once, on the mesh's own stream, and completed before any call (on any stream) uses it
+        if (int rc = build_gwn(h->m, h->m.stream)) return rc;
+        SSDF_HIP_TRY(hipStreamSynchronize(h->m.stream));
+    }
+    const QueryWs w = carve_ws(ws, (size_t)n, points_sorted(h->m, n));
+    return query_points_in(h->m, w, mode, n, x, y, z, out_d, out_tri, cx, cy, cz, flags, stream);
+}
 }  // namespace ssdf_amd
 
 extern "C" {
@@ -736,15 +790,9 @@ int ssdf_mesh_rebuild(ssdf_mesh_t mesh, int with_normals, void *stream) {
     }
     if (mesh->m.nsel <= 0) return SSDF_OK;
     const hipStream_t st = (hipStream_t)stream;
-    if (st != mesh->m.stream) {
-        // the handle's buffers are ordered on its creation stream
-        hipEvent_t ev;
-        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        SSDF_HIP_TRY(hipEventRecord(ev, mesh->m.stream));
-        SSDF_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
-        (void)hipEventDestroy(ev);
-        mesh->m.stream = st;
-    }
+    // the rebuild overwrites the tree and the triangle records: after every earlier build and query
+    if (int rc = order_after_handle(mesh, st)) return rc;
+    mesh->m.stream = st;
     mesh->m.has_normals = false;
     mesh->m.has_gwn = false;
     if (int rc = build_lbvh(mesh->m, st)) return rc;
@@ -762,6 +810,9 @@ int ssdf_mesh_update_vertices(ssdf_mesh_t mesh, const void *sx, const void *sy, 
     const size_t vb = (mesh->m.dtype ? sizeof(double) : sizeof(float)) * (size_t)mesh->m.nsp;
     const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipStream_t st = (hipStream_t)stream;
+    // queries still running on other streams read the vertex arrays (FP64 re-evaluation, normals):
+    // the copies wait for them
+    if (int rc = order_after_handle(mesh, st)) return rc;
     SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sx, sx, vb, kind, st));
     SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sy, sy, vb, kind, st));
     SSDF_HIP_TRY(hipMemcpyAsync(mesh->m.sz, sz, vb, kind, st));

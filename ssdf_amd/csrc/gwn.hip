@@ -26,9 +26,9 @@
 namespace ssdf_amd {
 namespace {
 
-constexpr int kScanBlock = 256;
+constexpr int kGwnScanBlock = 256;
 constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanBlock * kScanItems;
+constexpr int kGwnScanTile = kGwnScanBlock * kScanItems;
 constexpr int kCh = 8;  // area vector (3), area, area * centroid (3), unused
 constexpr float kInv4Pi = 0.07957747154594766788f;
 constexpr int kGwnWaves = 4;
@@ -55,12 +55,12 @@ __global__ __launch_bounds__(256) void k_gwn_leaf(const TriP *__restrict__ tris,
 
 // Inclusive scan of the kCh channels across the block's threads (one value set per thread).
 __device__ void block_scan8(double (&run)[kCh], double (&excl)[kCh], double (&total)[kCh]) {
-    __shared__ double s[kCh][kScanBlock];
+    __shared__ double s[kCh][kGwnScanBlock];
     const int t = threadIdx.x;
 #pragma unroll
     for (int c = 0; c < kCh; c++) s[c][t] = run[c];
     __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
+    for (int off = 1; off < kGwnScanBlock; off <<= 1) {
         double tmp[kCh];
 #pragma unroll
         for (int c = 0; c < kCh; c++) tmp[c] = t >= off ? s[c][t - off] : 0.0;
@@ -72,14 +72,14 @@ __device__ void block_scan8(double (&run)[kCh], double (&excl)[kCh], double (&to
 #pragma unroll
     for (int c = 0; c < kCh; c++) {
         excl[c] = t > 0 ? s[c][t - 1] : 0.0;
-        total[c] = s[c][kScanBlock - 1];
+        total[c] = s[c][kGwnScanBlock - 1];
     }
     __syncthreads();
 }
 
-// Local inclusive scan of one tile of kScanTile leaves; the tile's total goes to bsum.
-__global__ __launch_bounds__(kScanBlock) void k_scan8_local(double *__restrict__ v, int n, double *__restrict__ bsum) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+// Local inclusive scan of one tile of kGwnScanTile leaves; the tile's total goes to bsum.
+__global__ __launch_bounds__(kGwnScanBlock) void k_scan8_local(double *__restrict__ v, int n, double *__restrict__ bsum) {
+    const size_t base = (size_t)blockIdx.x * kGwnScanTile + (size_t)threadIdx.x * kScanItems;
     double run[kCh] = {};
     for (int i = 0; i < kScanItems; i++) {
         const size_t idx = base + i;
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan8_local(double *__restrict__
 }
 
 // Exclusive scan of the tile totals in place (one block).
-__global__ __launch_bounds__(kScanBlock) void k_scan8_tiles(double *__restrict__ bsum, int nt) {
+__global__ __launch_bounds__(kGwnScanBlock) void k_scan8_tiles(double *__restrict__ bsum, int nt) {
     double carry[kCh] = {};
-    for (int b0 = 0; b0 < nt; b0 += kScanBlock) {
+    for (int b0 = 0; b0 < nt; b0 += kGwnScanBlock) {
         const int i = b0 + (int)threadIdx.x;
         double run[kCh];
 #pragma unroll
@@ -127,9 +127,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan8_tiles(double *__restrict__
     }
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan8_add(double *__restrict__ v, int n,
+__global__ __launch_bounds__(kGwnScanBlock) void k_scan8_add(double *__restrict__ v, int n,
                                                           const double *__restrict__ boff) {
-    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    const size_t base = (size_t)blockIdx.x * kGwnScanTile + (size_t)threadIdx.x * kScanItems;
     const double *off = boff + (size_t)kCh * blockIdx.x;
     for (int i = 0; i < kScanItems; i++) {
         const size_t idx = base + i;
@@ -299,6 +299,12 @@ int dispatch_stack(const DevMesh &m, const QueryArgs &a, int mode, unsigned nblo
 
 }  // namespace
 
+size_t gwn_bytes_bound(ptrdiff_t nsel) {
+    const size_t n = (size_t)std::max<ptrdiff_t>(nsel, 1);
+    const size_t nt = (n + kGwnScanTile - 1) / kGwnScanTile;
+    return sizeof(GwnNode) * n + sizeof(double) * kCh * (n + nt);
+}
+
 // Far-field winding-number tree of the mesh (see the head of this file).
 int build_gwn(DevMesh &m, hipStream_t stream) {
     const int n = (int)m.nsel;
@@ -311,13 +317,13 @@ int build_gwn(DevMesh &m, hipStream_t stream) {
         SSDF_HIP_TRY(dev_alloc((void **)&m.gwn, sizeof(GwnNode) * (size_t)ni, stream));
         m.device_bytes += sizeof(GwnNode) * (size_t)ni;
     }
-    const int nt = (n + kScanTile - 1) / kScanTile;
+    const int nt = (n + kGwnScanTile - 1) / kGwnScanTile;
     if (int rc = ensure_ws(m, sizeof(double) * kCh * ((size_t)n + (size_t)nt), stream)) return rc;
     double *v = (double *)m.ws, *bsum = v + (size_t)kCh * n;
     hipLaunchKernelGGL(k_gwn_leaf, dim3((n + 255) / 256), dim3(256), 0, stream, m.tris, n, v);
-    hipLaunchKernelGGL(k_scan8_local, dim3(nt), dim3(kScanBlock), 0, stream, v, n, bsum);
-    hipLaunchKernelGGL(k_scan8_tiles, dim3(1), dim3(kScanBlock), 0, stream, bsum, nt);
-    hipLaunchKernelGGL(k_scan8_add, dim3(nt), dim3(kScanBlock), 0, stream, v, n, bsum);
+    hipLaunchKernelGGL(k_scan8_local, dim3(nt), dim3(kGwnScanBlock), 0, stream, v, n, bsum);
+    hipLaunchKernelGGL(k_scan8_tiles, dim3(1), dim3(kGwnScanBlock), 0, stream, bsum, nt);
+    hipLaunchKernelGGL(k_scan8_add, dim3(nt), dim3(kGwnScanBlock), 0, stream, v, n, bsum);
     hipLaunchKernelGGL(k_gwn_nodes, dim3((ni + 255) / 256), dim3(256), 0, stream, m.nodes, ni, v, m.gwn);
     SSDF_HIP_TRY(hipGetLastError());
     m.has_gwn = true;

@@ -415,6 +415,28 @@ __global__ __launch_bounds__(256) void k_finalize_nodes(const int *__restrict__ 
 
 static unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Build workspace of build_lbvh and build_normals for n triangles (the layouts below).
+static size_t lbvh_ws_bytes(size_t nn) {
+    const size_t ni = nn > 1 ? nn - 1 : 1;
+    return 16 * nn + 8 * nn + 24 * nn + 8 * nn + 48 * ni + 8 * ni + 48 * (size_t)nblk(nn, 256) +
+           4 * radix_scratch_words(nn);
+}
+static size_t normals_ws_bytes(size_t nn) { return 48 * 3 * nn + 4 * radix_scratch_words(3 * nn); }
+
+size_t mesh_bytes_bound(ptrdiff_t nsel, ptrdiff_t nsp, int dtype, bool with_normals) {
+    const size_t n = (size_t)std::max<ptrdiff_t>(nsel, 1), V = (size_t)std::max<ptrdiff_t>(nsp, 1);
+    const size_t ni = n > 1 ? n - 1 : 1;
+    const size_t vb = dtype ? sizeof(double) : sizeof(float);
+    size_t b = 3 * sizeof(int) * n + 3 * vb * V;                     // caller-layout copies
+    b += sizeof(Node) * ni + sizeof(TriP) * n + sizeof(MeshMeta);   // LBVH
+    size_t ws = lbvh_ws_bytes(n);
+    if (with_normals) {
+        b += sizeof(double) * (12 * n + 3 * V);
+        ws = std::max(ws, normals_ws_bytes(n));
+    }
+    return b + ws;
+}
+
 int build_lbvh(DevMesh &m, hipStream_t stream) {
     const int n = (int)m.nsel;
     if (m.nsel <= 0 || m.nsel >= ((ptrdiff_t)1 << kLeafBits)) {

@@ -61,6 +61,10 @@ int build_normals(DevMesh &m, hipStream_t stream);
 // Far-field tree of the generalized winding number (gwn.hip): per internal node, each child's
 // area vector, centroid and radius, from FP64 prefix sums over the Morton-ordered leaves.
 int build_gwn(DevMesh &m, hipStream_t stream);
+// Upper bounds of the device bytes a mesh can hold: its arrays, LBVH, normals (with_normals) and
+// build workspace (lbvh.hip); the winding-number tree and its build workspace (gwn.hip).
+size_t mesh_bytes_bound(ptrdiff_t nsel, ptrdiff_t nsp, int dtype, bool with_normals);
+size_t gwn_bytes_bound(ptrdiff_t nsel);
 // Copies meta back (synchronises the stream).
 int sync_meta(DevMesh &m, hipStream_t stream);
 // Frees the mesh's device arrays, ordered on `st` (a stream that is still alive; null: the null

@@ -396,7 +396,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         if (active && rank < got) {
             const unsigned slot = base + rank;
             a.fix_list[4 * slot] = (unsigned)qid;
-            a.fix_list[4 * slot + 1] = __float_as_uint(FIX ? band_sq(bestd, tau1, babs) : best);
+            // the bound handed over is the FP32 band of the best so far (plain mode too: the
+            // per-query walk then finds that triangle again even where an ancestor box's FP32
+            // distance rounds above the triangle's)
+            a.fix_list[4 * slot + 1] = __float_as_uint(band_sq(FIX ? bestd : best, tau1, babs));
             a.fix_list[4 * slot + 2] = a.fix_tag;
             a.fix_list[4 * slot + 3] = (unsigned)((unsigned long long)qid >> 32);
             active = false;
@@ -803,8 +806,8 @@ __device__ bool coop_exact(const QueryArgs &a, const long long qid, const double
     return true;
 }
 
-// Plain entry: the FP32 minimum at or below bound0 (the packet wave's best so far), written by the
-// packet epilogue.
+// Plain entry: the FP32 minimum at or below bound0 (the FP32 band of the packet wave's best so far),
+// written by the packet epilogue.
 template <int QSRC, int OUT>
 __device__ bool coop_plain(const QueryArgs &a, const long long qid, const float bound0, int *lev, const int nlev) {
     double dx, dy, dz;
@@ -1077,11 +1080,6 @@ __global__ __launch_bounds__(256) void k_point_keys(long long n, const float *__
     }
     keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
     vals[i] = (uint32_t)i;
-}
-
-size_t sort_ws_bytes(long long n) {
-    const size_t nn = (size_t)std::max<long long>(n, 1);
-    return 16 * nn + 4 * radix_scratch_words(nn) + 256;
 }
 
 int sort_points(const MeshMeta *meta, long long n, const float *x, const float *y, const float *z, void *ws,

@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "mesh.hpp"
+#include "radix.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -149,7 +150,56 @@ int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out,
 // 30-bit Morton code of their position in the mesh's root box (4 radix passes) and the waves take
 // them through the resulting permutation; results still land at the points' own positions.
 constexpr long long kSortMin = 4096;
-size_t sort_ws_bytes(long long n);
+// Workspace of sort_points for n points: keys and values twice (ping-pong), the radix tile table
+// and digit totals, and a margin.
+constexpr size_t sort_ws_bytes(long long n) {
+    return 16 * (size_t)(n > 1 ? n : 1) + 4 * radix_scratch_words((size_t)(n > 1 ? n : 1)) + 256;
+}
+// The device memory one query call works in, wherever it lives (a handle's workspace, a serving
+// request's buffer, a static array): status words ([0] queue entries written, [1] kernel error
+// flag, [2] queue read position), the queue of 4-word entries, and the Morton sort of the points
+// (FP32 point sets of kSortMin points or more).
+struct QueryWs {
+    unsigned *status = nullptr;
+    unsigned *list = nullptr;
+    size_t entries = 0;
+    void *sort = nullptr;
+    size_t sort_bytes = 0;
+};
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Queue entries reserved for nq queries (FIX near-ties that overflow the candidate list, and the
+// queries of packet waves past their leaf budget): an eighth of the queries, at least 64 Ki.  Past
+// that a near-tie keeps its FP32 result and sets bit 4 of the kernel error flag, and a heavy query
+// stays in its packet wave.
+constexpr size_t fixup_entries(size_t nq) { return std::min(nq, std::max<size_t>(1u << 16, nq / 8)); }
+
+// Bytes of a contiguous QueryWs for nq queries (sorted: with room for the point sort).
+constexpr size_t query_ws_bytes(size_t nq, bool sorted) {
+    return 256 + round256(16 * std::max<size_t>(fixup_entries(nq), 1)) +
+           (sorted ? round256(sort_ws_bytes((long long)nq)) : 0);
+}
+
+// Splits a 256-byte aligned buffer of query_ws_bytes(nq, sorted) bytes into a QueryWs.
+inline QueryWs carve_ws(void *base, size_t nq, bool sorted) {
+    QueryWs w;
+    char *p = (char *)base;
+    w.status = (unsigned *)p;
+    w.list = (unsigned *)(p + 256);
+    w.entries = std::max<size_t>(fixup_entries(nq), 1);
+    if (sorted) {
+        w.sort = p + 256 + round256(16 * w.entries);
+        w.sort_bytes = round256(sort_ws_bytes((long long)nq));
+    }
+    return w;
+}
+
+// A point query on a resident mesh in the caller's workspace `ws` (capi.hip).
+int ws_query_points(ssdf_mesh_t h, void *ws, bool may_build_gwn, int mode, ptrdiff_t n, const void *x,
+                    const void *y, const void *z, void *out_d, int *out_tri, void *cx, void *cy, void *cz,
+                    unsigned flags, hipStream_t stream);
+
 // perm (in ws) = the Morton order of the n points; ws holds sort_ws_bytes(n) bytes.
 int sort_points(const MeshMeta *meta, long long n, const float *x, const float *y, const float *z, void *ws,
                 const unsigned **perm, hipStream_t stream);

@@ -32,10 +32,10 @@ constexpr int kScanBlock = 256;    // zero and row-scan blocks: one thread per d
 constexpr unsigned kFuseTiles = 192;
 constexpr int kMaxPasses = 8;
 
-inline unsigned radix_tiles(size_t n) { return (unsigned)((n + kRadixTile - 1) / kRadixTile); }
+constexpr unsigned radix_tiles(size_t n) { return (unsigned)((n + kRadixTile - 1) / kRadixTile); }
 
 // scratch: the (digit, tile) table, then 256 digit totals per pass
-inline size_t radix_scratch_words(size_t n) { return (size_t)256 * radix_tiles(n) + 256 * kMaxPasses + 64; }
+constexpr size_t radix_scratch_words(size_t n) { return (size_t)256 * radix_tiles(n) + 256 * kMaxPasses + 64; }
 
 template <typename T>
 __global__ __launch_bounds__(kScanBlock) void k_radix_zero(T *__restrict__ p, unsigned n) {

@@ -4,12 +4,16 @@
 //   * Mesh cache: registered meshes (host arrays, kept by the caller) are uploaded and built on
 //     first use and stay resident; when an admission needs room, the least recently used meshes
 //     that no in-flight request references are evicted.
-//   * Admission by HBM budget: a request is admitted (FIFO) when its device buffers (queries in,
-//     distances and triangles out: 20 B per query) plus any mesh it needs fit in the budget next
-//     to what is resident and in flight; otherwise it waits for completions.
+//   * Admission by HBM budget: a request is admitted (FIFO) when its device buffer (queries in,
+//     distances and triangles out: 20 B per query, plus its own query workspace: the fixup queue
+//     and the Morton sort of its points) plus any mesh it needs fit in the budget next to what is
+//     resident and in flight; otherwise it waits for completions.
 //   * Continuous batching: admitted requests go round-robin onto a small pool of streams (upload,
 //     query, download, completion event) as soon as they are admitted, without batch boundaries;
-//     a worker thread admits, launches and retires.
+//     a worker thread admits, launches and retires.  Each request works in its own buffer, so
+//     requests on the same mesh overlap on the device too.
+//   * Meshes load on a stream of their own: a load waits for its build only, not for the requests
+//     queued on the serving streams.
 //   * Latency = completion (results in host memory) minus arrival (the submit time, or an arrival
 //     timestamp on the server clock given by the caller); statistics keep every request's latency.
 #include <algorithm>
@@ -92,6 +96,7 @@ struct ssdf_server {
     std::unordered_map<uint64_t, Request *> by_ticket;
     uint64_t next_ticket = 1;
     hipStream_t streams[kStreams] = {};
+    hipStream_t load_stream = nullptr;  // mesh uploads and builds
     int next_stream = 0;
     int max_inflight = 16;
     bool stop = false;
@@ -133,12 +138,18 @@ bool make_room(ssdf_server *s, size_t need) {
     return true;
 }
 
-size_t estimate_mesh_bytes(const MeshEntry &e) {
-    // LBVH nodes, triangle records, normals, inputs and the build workspace (mesh.hpp layout)
-    return (size_t)e.nsel * (64 + 144 + 96 + 12 + 120) + (size_t)e.nsp * (12 + 24) + (1u << 16);
+// What a resident mesh is charged: an upper bound of everything it can allocate (arrays, LBVH,
+// pseudonormals, build workspace, and the winding-number tree that winding-number signs build on
+// first use), so that admission never lets the device memory in use pass the budget.
+size_t mesh_charge(const MeshEntry &e) {
+    return mesh_bytes_bound(e.nsel, e.nsp, 0, true) + gwn_bytes_bound(e.nsel);
 }
 
-size_t request_bytes(const Request *r) { return (size_t)r->n * 20 + 256; }
+// Device bytes of a request: points in, distances and triangles out, then its query workspace.
+size_t request_io_bytes(const Request *r) { return round256((size_t)r->n * 20); }
+size_t request_bytes_of(const Request *r) {
+    return request_io_bytes(r) + query_ws_bytes((size_t)r->n, r->n >= kSortMin);
+}
 
 // Admits the head of the queue if it fits; returns false when it has to wait.
 bool admit_one(ssdf_server *s, std::unique_lock<std::mutex> &lk) {
@@ -146,8 +157,8 @@ bool admit_one(ssdf_server *s, std::unique_lock<std::mutex> &lk) {
     Request *r = s->pending.front();
     auto mit = s->meshes.find(r->mesh);
     MeshEntry &e = mit->second;
-    const size_t rb = request_bytes(r);
-    const size_t mb = e.resident ? 0 : estimate_mesh_bytes(e);
+    const size_t rb = request_bytes_of(r);
+    const size_t mb = e.resident ? 0 : mesh_charge(e);
     if (!make_room(s, rb + mb)) return false;
     s->pending.pop_front();
     const int slot = s->next_stream;
@@ -155,14 +166,15 @@ bool admit_one(ssdf_server *s, std::unique_lock<std::mutex> &lk) {
     hipStream_t st = s->streams[slot];
     int rc = SSDF_OK;
     if (!e.resident) {
-        // build on the request's stream; the mesh's later users are ordered after it by the handle
+        // upload and build on the load stream; the build is complete when the call returns
         lk.unlock();
         size_t bytes = 0;
-        rc = server_mesh_create(e.s0, e.s1, e.s2, e.nsel, e.sx, e.sy, e.sz, e.nsp, st, &e.handle, &bytes);
+        rc = server_mesh_create(e.s0, e.s1, e.s2, e.nsel, e.sx, e.sy, e.sz, e.nsp, s->load_stream, &e.handle,
+                                &bytes);
         lk.lock();
         if (!rc) {
             e.resident = true;
-            e.bytes = std::max(bytes, mb);
+            e.bytes = mb;  // the bound, not what the build has allocated so far
             s->used += e.bytes;
             s->lru.push_front(r->mesh);
             e.lru = s->lru.begin();
@@ -194,8 +206,8 @@ bool admit_one(ssdf_server *s, std::unique_lock<std::mutex> &lk) {
                 he = hipMemcpyAsync(dt, r->out_tri, 4 * n, hipMemcpyHostToDevice, st);
             if (he != hipSuccess) rc = SSDF_ERR_HIP;
             if (!rc)
-                rc = ssdf_mesh_query_points(e.handle, r->mode, r->n, dx, dy, dz, dd, r->out_tri ? dt : nullptr,
-                                            nullptr, nullptr, nullptr, r->flags, st);
+                rc = ws_query_points(e.handle, d + request_io_bytes(r), true, r->mode, r->n, dx, dy, dz, dd,
+                                     r->out_tri ? dt : nullptr, nullptr, nullptr, nullptr, r->flags, st);
             if (!rc) {
                 he = hipMemcpyAsync(r->out_d, dd, 4 * n, hipMemcpyDeviceToHost, st);
                 if (he == hipSuccess && r->out_tri) he = hipMemcpyAsync(r->out_tri, dt, 4 * n, hipMemcpyDeviceToHost, st);
@@ -286,8 +298,9 @@ int ssdf_server_create(size_t hbm_budget_bytes, int max_inflight, ssdf_server_t 
     s->budget = hbm_budget_bytes;
     if (max_inflight > 0) s->max_inflight = max_inflight;
     (void)hipGetDevice(&s->device);
-    for (int i = 0; i < kStreams; i++) {
-        if (hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking) != hipSuccess) {
+    for (int i = 0; i <= kStreams; i++) {
+        hipStream_t *sp = i < kStreams ? &s->streams[i] : &s->load_stream;
+        if (hipStreamCreateWithFlags(sp, hipStreamNonBlocking) != hipSuccess) {
             for (int j = 0; j < i; j++) (void)hipStreamDestroy(s->streams[j]);
             delete s;
             set_error("ssdf_server_create: stream creation failed");
@@ -312,6 +325,24 @@ int ssdf_server_add_mesh(ssdf_server_t s, int mesh_id, ptrdiff_t nselements, con
     }
     MeshEntry e{s0, s1, s2, sx, sy, sz, nselements, nspoints};
     s->meshes.emplace(mesh_id, e);
+    return SSDF_OK;
+}
+
+int ssdf_server_charge(ssdf_server_t s, int mesh_id, ptrdiff_t npoints, size_t *mesh_bytes, size_t *request_bytes) {
+    if (!s || npoints < 0) {
+        set_error("ssdf_server_charge: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->meshes.find(mesh_id);
+    if (it == s->meshes.end()) {
+        set_error("ssdf_server_charge: unknown mesh id");
+        return SSDF_ERR_INVALID;
+    }
+    Request r{};
+    r.n = npoints;
+    if (mesh_bytes) *mesh_bytes = mesh_charge(it->second);
+    if (request_bytes) *request_bytes = request_bytes_of(&r);
     return SSDF_OK;
 }
 
@@ -343,7 +374,7 @@ int ssdf_server_submit(ssdf_server_t s, int mesh_id, int mode, ptrdiff_t npoints
             set_error("ssdf_server_submit: unknown mesh id");
             return SSDF_ERR_INVALID;
         }
-        if (request_bytes(r) + (it->second.resident ? 0 : estimate_mesh_bytes(it->second)) > s->budget) {
+        if (request_bytes_of(r) + (it->second.resident ? 0 : mesh_charge(it->second)) > s->budget) {
             delete r;
             set_error("ssdf_server_submit: request larger than the HBM budget");
             return SSDF_ERR_OOM;
@@ -438,6 +469,7 @@ int ssdf_server_destroy(ssdf_server_t s) {
         if (kv.second.handle) server_mesh_destroy(kv.second.handle);
     for (auto &kv : s->by_ticket) delete kv.second;
     for (int i = 0; i < kStreams; i++) (void)hipStreamDestroy(s->streams[i]);
+    (void)hipStreamDestroy(s->load_stream);
     delete s;
     return SSDF_OK;
 }

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import ssdf_amd as S
+from tests import bars
 from ssdf_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
@@ -82,9 +83,9 @@ def test_batch_mesh_against_oracle(meshes, oracle):
     N = int(n.prod())
     assert np.array_equal(tri.download()[off[i]:off[i] + N], rtri)
     lo, hi = m.bbox()
-    eps = 1e-6 * float(np.linalg.norm(hi - lo))
     o = out.download()[off[i]:off[i] + N]
-    assert np.abs(np.abs(o) - np.abs(s)).max() <= eps
+    bars.assert_fixup_distances(o, np.abs(s), "batch mesh 3")
+    assert bars.sign_mismatches_outside_band(o, s, 1e-6 * float(np.linalg.norm(hi - lo))) == 0
 
 
 def test_batch_points_closest_and_edf(meshes):

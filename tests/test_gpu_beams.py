@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import ssdf_amd as S
+from tests import bars
 from ssdf_amd import meshgen as mg
 from ssdf_amd import workloads as W
 
@@ -30,9 +31,9 @@ def test_beams_closed_against_oracle(oracle):
     x, y, z = mg.grid_points(n, n, n, (0, 0, 0), (1, 1, 1))
     s, rtri, _, _, _ = oracle.sdf(x, y, z, m)
     assert np.array_equal(t, rtri)
-    eps = 1e-6 * np.sqrt(3.0)
-    assert np.abs(np.abs(o) - np.abs(s)).max() <= eps
-    assert np.sum((np.sign(o) != np.sign(s)) & (np.abs(s) > eps)) == 0
+    bars.assert_fixup_distances(o, np.abs(s), "beams grid")
+    lo, hi = m.bbox()
+    assert bars.sign_mismatches_outside_band(o, s, 1e-6 * float(np.linalg.norm(hi - lo))) == 0
     assert np.any(o < 0)  # some nodes fall inside a beam
 
 

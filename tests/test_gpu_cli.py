@@ -9,8 +9,21 @@ import pytest
 
 from ssdf_amd import meshgen as mg
 from ssdf_amd import rawio
+from tests import bars
 
 pytestmark = pytest.mark.gpu
+
+
+def _union_diag():
+    """Bounding-box diagonal of the scene's two surfaces (the sphere and the shifted torus)."""
+    a = mg.uv_sphere(0.5, 32, 16)
+    b = mg.torus(0.6, 0.2, 60, 24, sigma=0.003, seed=4)
+    lo = np.minimum(a.bbox()[0], b.bbox()[0] + [1.5, 0, 0])
+    hi = np.maximum(a.bbox()[1], b.bbox()[1] + [1.5, 0, 0])
+    return float(np.linalg.norm(hi - lo))
+
+
+DIAG = _union_diag()
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +56,7 @@ def test_union_of_surfaces_fp32(scene):
     d, _, da, db = scene
     _run(d, d / "a", d / "b", d / "p", d / "o.raw")
     out = rawio.read_field(d / "o.raw")
-    assert np.abs(out - np.minimum(da, db)).max() <= 1e-6 * 4.0
+    bars.plain_report(out, np.minimum(da, db), DIAG, "sdf command")
     # .raw file names give the same field
     _run(d, d / "a", d / "b_raw", d / "p", d / "o2.raw")
     assert np.array_equal(rawio.read_field(d / "o2.raw"), out)
@@ -62,10 +75,10 @@ def test_warm_start_and_volume_box(scene, oracle):
     seed = np.full(x.size, 0.05, np.float32)
     seed.tofile(d / "seed.raw")
     _run(d, d / "a", d / "p", d / "ow.raw", SSDF_INPUT_SDF=d / "seed.raw")
-    assert np.abs(rawio.read_field(d / "ow.raw") - np.minimum(0.05, da)).max() <= 1e-6 * 4.0
+    bars.plain_report(rawio.read_field(d / "ow.raw"), np.minimum(np.float64(np.float32(0.05)), da), DIAG, "warm start")
     _run(d, d / "b", d / "p", d / "ov.raw", SSDF_INIT_WITH_VOLUME_BOX=1)
     box = np.abs(oracle.all_points_aabb_sdist(x, y, z, oracle.compute_aabb(x, y, z)))
-    assert np.abs(rawio.read_field(d / "ov.raw") - np.minimum(box, db)).max() <= 1e-6 * 4.0
+    bars.plain_report(rawio.read_field(d / "ov.raw"), np.minimum(box, db), DIAG, "volume box")
 
 
 def test_wrong_seed_size_fails(scene):
@@ -114,7 +127,7 @@ def test_mesh_to_sdf(tmp_path, oracle):
     s, _, _, _, _ = oracle.sdf(x, y, z, m)
     diag = float(np.linalg.norm(np.array(bhi, np.float64) - np.array(blo, np.float64)))
     got = -field.reshape(-1).astype(np.float64)
-    assert np.abs(np.abs(got) - np.abs(s)).max() <= 1e-6 * diag
+    bars.plain_report(got, np.abs(s), diag, "mesh_to_sdf")
     far = np.abs(s) > 1e-6 * diag
     assert np.array_equal(np.sign(got[far]), np.sign(s[far]))
 
@@ -163,7 +176,7 @@ def test_points_inside_and_sdf_sign_chain(scene, oracle):
     run(rawio.SDF_SIGN_EXE, d / "u.raw", d / "in.raw", d / "s.raw")
     signed = np.fromfile(d / "s.raw", np.float32)
     udf = np.minimum(da, db)
-    assert np.abs(np.abs(signed) - udf).max() <= 1e-6 * 4.0
+    bars.plain_report(signed, udf, DIAG, "sdf_sign chain")
     assert np.array_equal(signed < 0, (inside_want == 1) & (udf > 0))
 
 

@@ -73,11 +73,19 @@ def test_served_results_equal_direct(pool):
 
 
 def test_budget_evicts_idle_meshes(pool):
-    # room for one 8k-triangle mesh (about 4 MiB with its build workspace) plus a request
-    srv = S.Server(6 << 20, max_inflight=2)
+    reqs = _requests(pool, 12, 2)
+    # room for the largest mesh plus the largest request, not for two meshes
+    probe = S.Server(1 << 40)
+    for i, m in enumerate(pool):
+        probe.add_mesh(i, m)
+    charges = sorted(probe.charge(i, 1)[0] for i in range(len(pool)))
+    req_b = max(probe.charge(0, r[3][0].size)[1] for r in reqs)
+    probe.close()
+    budget = charges[-1] + req_b + 4096
+    assert budget < charges[-1] + charges[-2]  # the two larger meshes never fit together
+    srv = S.Server(budget, max_inflight=2)
     for i, m in enumerate(pool):
         srv.add_mesh(i, m)
-    reqs = _requests(pool, 12, 2)
     res = []
     for mid, mode, flags, (x, y, z), init in reqs:
         o, t = init.copy(), np.full(x.size, -1, np.int32)
@@ -89,7 +97,7 @@ def test_budget_evicts_idle_meshes(pool):
         assert np.array_equal(o, d) and np.array_equal(t, tt)
     st = srv.stats()
     assert st["evictions"] > 0 and st["mesh_loads"] > len(pool) - 1
-    assert st["peak_bytes"] <= 6 << 20
+    assert st["peak_bytes"] <= budget
     srv.close()
 
 
