@@ -198,6 +198,21 @@ int ssdf_mesh_query_points(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const 
                            const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy, void *out_cz,
                            unsigned flags, void *stream);
 
+/* Allocation-free variant of ssdf_mesh_query_points.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as ssdf_mesh_query_points, but the
+ * call's workspace (status words, queue, point sort) is a static device array of the library, about
+ * 18 MiB on each GPU that uses it, held until the process ends; the call itself allocates nothing.
+ * npoints <= SSDF_NOALLOC_MAX_POINTS, else SSDF_ERR_INVALID.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (any mesh, any stream) race, so the caller serializes them.
+ * It is ordered only by `stream` (no ordering against the handle's other streams).  Signed queries
+ * signed by winding numbers (SSDF_FLAG_SIGN_GWN / AUTO) need the mesh's winding-number tree built by
+ * an earlier allocating query; ssdf_mesh_last_stats does not report these calls. */
+#define SSDF_NOALLOC_MAX_POINTS 1048576
+int ssdf_mesh_query_points_noalloc(ssdf_mesh_t mesh, int mode, ptrdiff_t npoints, const void *x, const void *y,
+                                   const void *z, void *out_d, int *out_tri, void *out_cx, void *out_cy,
+                                   void *out_cz, unsigned flags, void *stream);
+
 /* Grid nodes x_i = min + i (max - min) / (n - 1), computed in the mesh precision.  Modes EDF, SDF
  * or SQEDF. */
 int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin,

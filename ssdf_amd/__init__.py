@@ -30,6 +30,7 @@ FLAG_FIXUP = 1
 FLAG_SIGN_FP64 = 2
 FLAG_SIGN_GWN = 8
 FLAG_SIGN_AUTO = 16
+NOALLOC_MAX_POINTS = 1 << 20  # SSDF_NOALLOC_MAX_POINTS
 MODE_EDF, MODE_SDF, MODE_CLOSEST, MODE_SQEDF = 0, 1, 2, 3
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libssdf_amd.so"
@@ -88,6 +89,7 @@ _SIGS = {
     "ssdf_mesh_update_vertices": [_P, _P, _P, _P, C.c_int, C.c_int, _P],
     "ssdf_mesh_info": [_P, C.POINTER(_MeshInfo)],
     "ssdf_mesh_query_points": [_P, C.c_int, _PD, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
+    "ssdf_mesh_query_points_noalloc": [_P, C.c_int, _PD, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
     "ssdf_mesh_query_grid": [_P, C.c_int, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_double, C.c_double, _P, _P, C.c_uint, _P],
     "ssdf_mesh_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
@@ -470,6 +472,15 @@ class ResidentMesh:
                                           _ptr(out_tri), _ptr(cx), _ptr(cy), _ptr(cz), flags,
                                           C.c_void_p(stream.ptr) if stream else None)
         _check(rc, "ssdf_mesh_query_points")
+
+    def query_points_noalloc(self, mode, x, y, z, out_d, out_tri=None, cx=None, cy=None, cz=None, flags=0,
+                             stream: Stream | None = None):
+        """query_points in the library's static workspace (no allocation; at most NOALLOC_MAX_POINTS
+        points; calls that overlap on one GPU race)."""
+        rc = lib().ssdf_mesh_query_points_noalloc(C.c_void_p(self.h), mode, x.n, _ptr(x), _ptr(y), _ptr(z),
+                                                  _ptr(out_d), _ptr(out_tri), _ptr(cx), _ptr(cy), _ptr(cz), flags,
+                                                  C.c_void_p(stream.ptr) if stream else None)
+        _check(rc, "ssdf_mesh_query_points_noalloc")
 
     def query_grid(self, mode, n, lo, hi, out_d, out_tri=None, flags=0, stream: Stream | None = None):
         nx, ny, nz = n
