@@ -7,18 +7,30 @@ displaced along its normal by N(0, 0.005^2) (SplitMix64 seed 2); query set: the 
 
 A step is one full job on inputs already resident in HBM: LBVH build + FP64 pseudonormals from the
 device mesh arrays, then the 256^3 signed query in FP32 + FP64 fixup, outputs written to HBM.
-`value` = queries / step time.  Also reported: the same step in plain FP32 (no fixup), the query
-alone on a resident BVH, the reference-shaped host call (upload, build, query, download), the
-roofline of the query kernel and the CPU baseline (the oracle's restatement of
-edf_celllist<float>, timed on this host on a seeded subsample of the same grid).
+`value` = queries / step time (device-resident).  Also reported: the same step in plain FP32, the
+query alone on a resident BVH, the reference-shaped host calls (upload, build, query, download) in
+plain FP32 and with the fixup, the roofline of the query kernel, and the other workloads:
+  batch64   64 displaced icospheres (1 k - 500 k triangles) with their own grids (84.8 M nodes),
+            one persistent launch per call, against 64 reference-shaped per-mesh calls;
+  served    Poisson arrivals (200 requests/s for 60 s) over a pool of 32 resident meshes
+            (5 k - 2 M triangles), 1e4 - 1e7 distinct points per request, plus a sweep of lower
+            arrival rates for the highest one served with bounded latency;
+  beams200  200 capped cylinders (4.99 M triangles), 512^3 grid and 1e8 scattered points, intact
+            and with 1% of the triangles deleted (winding-number signs).
+The cpu_baseline leg times the oracle's restatement of edf_celllist<float> (the reference's CPU
+engine) on the host's physical cores, and is where every leg's sampled results are checked against
+the oracle's exact FP64 argmin: each leg reports its max and RMS distance error, index mismatches
+and sign mismatches outside the band (`error_vs_fp64`).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -30,10 +42,42 @@ METRIC = "Mqueries/s @256^3×200k tris (FP32+fixup vs plain); served p50/p99 lat
 FLOPS_BOX, FLOPS_TRI, FLOPS_SIGN = 8, 81, 8
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+HOST_THREADS_MAX = 16  # the GPU box's CPU share
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def _rnd(v, k=4):
+    return round(float(v), k)
+
+
+class Samples:
+    """Seeded subsamples of every leg's results, kept for the FP64 check of the cpu_baseline leg.
+    Each entry: leg name, mesh, FP32 query coordinates, the GPU's (signed) distance and triangle,
+    and how its sign is judged (the mesh's own pseudonormal sign outside the 1e-6 * bbox band, or
+    another mesh's sign outside a given band)."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, leg, mesh, x, y, z, d, tri, signed=True, sign_mesh=None, sign_band=None):
+        self.items.append(dict(leg=leg, mesh=mesh, x=np.ascontiguousarray(x, np.float32),
+                               y=np.ascontiguousarray(y, np.float32), z=np.ascontiguousarray(z, np.float32),
+                               d=np.asarray(d, np.float32).copy(), tri=np.asarray(tri, np.int32).copy(),
+                               signed=signed, sign_mesh=sign_mesh, sign_band=sign_band))
+
+
+def _pick(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, size=min(k, n), replace=False))
+
+
+def _grid_coords(idx, n, lo, hi):
+    from ssdf_amd import meshgen as mg
+    ax = [mg.grid_axis(n[d], lo[d], hi[d]) for d in range(3)]
+    i, j, k = idx % n[0], (idx // n[0]) % n[1], idx // (n[0] * n[1])
+    return ax[0][i], ax[1][j], ax[2][k]
 
 
 def make_workload(args):
@@ -43,7 +87,7 @@ def make_workload(args):
     return m, n, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
 
 
-def run_gpu(args, m, n, lo, hi):
+def run_gpu(args, m, n, lo, hi, samples):
     import ctypes as C
 
     import ssdf_amd as S
@@ -85,6 +129,8 @@ def run_gpu(args, m, n, lo, hi):
     modes = (("fixup", S.FLAG_FIXUP), ("plain", 0))
     if args.only:
         modes = tuple(md for md in modes if md[0] == args.only)
+    idx = _pick(N, args.check_c2, seed=12)
+    sx, sy, sz = _grid_coords(idx, n, lo, hi)
     for name, flags in modes:
         qt = []
         for _ in range(args.warmup):
@@ -99,6 +145,8 @@ def run_gpu(args, m, n, lo, hi):
         res[name] = {"ms_per_step": dt * 1e3, "query_ms": float(np.mean(qt)), "query_ms_min": float(np.min(qt)),
                      "mqps": N / dt / 1e6, "query_mqps": N / (np.mean(qt) * 1e-3) / 1e6}
         log(f"[bench] {name}: step {dt * 1e3:.3f} ms, query {np.mean(qt):.3f} ms")
+        if not args.only:
+            samples.add(f"c2_{name}", m, sx, sy, sz, out.download()[idx], tri.download()[idx])
     if args.only:
         return res
     # fixup statistics of one more fixup query on the resident mesh
@@ -127,14 +175,22 @@ def run_gpu(args, m, n, lo, hi):
     res["build_ms"] = float(np.min(bt))
     # the reference-shaped host call: upload, build, query, download per call (serial/bvh/bvh.cu:80-158)
     outh = np.empty(N, np.float32)
-    t0 = time.perf_counter()
-    S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, n[0], n[1], n[2], *lo, *hi, outh, None, flags=0)
-    res["host_call_plain_s"] = time.perf_counter() - t0
-    res["host_call_plain_mqps"] = N / res["host_call_plain_s"] / 1e6
+    trih = np.empty(N, np.int32)
+    for key, flags in (("plain", 0), ("fixup", S.FLAG_FIXUP)):
+        S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, n[0], n[1], n[2], *lo, *hi, outh, trih, flags=flags)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, n[0], n[1], n[2], *lo, *hi, outh, trih,
+                             flags=flags)
+            ts.append(time.perf_counter() - t0)
+        res[f"host_call_{key}_s"] = float(np.min(ts))
+        res[f"host_call_{key}_mqps"] = N / res[f"host_call_{key}_s"] / 1e6
+        log(f"[bench] host call {key}: {res[f'host_call_{key}_s'] * 1e3:.1f} ms")
     return res
 
 
-def run_batch(args):
+def run_batch(args, samples):
     """64 displaced icospheres (4.1 M triangles), one grid each (84.8 M nodes), signed distance +
     triangle with FP64 fixup: the per-mesh reference-shaped call vs one batch."""
     import ssdf_amd as S
@@ -145,6 +201,9 @@ def run_batch(args):
     res = {"meshes": len(meshes), "triangles": int(sum(m.ntris for m in meshes)), "queries": nq}
     # per-mesh plain path: upload, build, query, download, free per call (serial/bvh/bvh.cu:80-158)
     outs = [np.empty(int(d.prod()), np.float32) for d in dims]
+    for m, d, bx, o in zip(meshes[:2], dims, boxes, outs):  # driver and code paths warm
+        S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, int(d[0]), int(d[1]), int(d[2]), *bx.astype(np.float32),
+                         o, None, flags=flags)
     t0 = time.perf_counter()
     for m, d, bx, o in zip(meshes, dims, boxes, outs):
         S.sample_sdf_bvh(m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, int(d[0]), int(d[1]), int(d[2]), *bx.astype(np.float32),
@@ -189,38 +248,103 @@ def run_batch(args):
     res["fixup_queued"], res["kernel_error"] = q, e
     res["speedup_resident_step_vs_per_mesh_calls"] = res["resident_step_mqps"] / res["per_mesh_calls_mqps"]
     res["speedup_host_to_host_vs_per_mesh_calls"] = res["batch_host_to_host_mqps"] / res["per_mesh_calls_mqps"]
+    ho, ht = out.download(), tri.download()
+    for i, m in enumerate(meshes):
+        nd = tuple(int(v) for v in dims[i])
+        idx = _pick(int(np.prod(nd)), args.check_batch, seed=100 + i)
+        x, y, z = _grid_coords(idx, nd, boxes[i][:3], boxes[i][3:])
+        samples.add("batch64", m, x, y, z, ho[off[i] + idx], ht[off[i] + idx])
     b.close()
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res.items()}
 
 
-def run_serve(args):
-    """Open-loop serving: Poisson arrivals (200 requests/s) over a pool of 32 resident meshes
-    (5 k - 2 M triangles), 1e4 - 1e7 signed queries per request (half uniform in the padded bbox,
-    half within 0.02 of the surface), FP32 + fixup; latency from the scheduled arrival to the
-    results in host memory."""
+def _pool_fill(mesh, arrays, npts, seed):
+    """Fills three pinned arrays with `npts` query points of `mesh` (workloads.query_cloud) and
+    returns how many of them are distinct (a 64-bit hash of the three coordinates' bits; a hash
+    collision can only undercount)."""
+    from ssdf_amd import workloads as W
+    pts = W.query_cloud(mesh, npts, seed=seed)
+    for a, p in zip(arrays, pts):
+        np.copyto(a.array, p)
+    b = [p.view(np.uint32).astype(np.uint64) for p in pts]
+    h = (b[0] * np.uint64(0x9E3779B97F4A7C15)) ^ (b[1] * np.uint64(0xC2B2AE3D27D4EB4F)) ^ \
+        (b[2] * np.uint64(0x165667B19E3779F9))
+    return int(np.unique(h).size)
+
+
+def _serve_window(srv, S, meshes, pools, outs, times, mesh_ids, nq, offs, t0, on_done=None, tag="serve"):
+    """Submits the requests at their scheduled arrivals (t0 + times on the server clock) and waits
+    for all; output buffers are a ring, so a slot's earlier request is waited for before reuse
+    (latency still counts from the scheduled arrival).  Returns per-request latencies."""
+    ring = len(outs)
+    slots = [None] * ring
+    lat = np.zeros(len(times))
+    for k in range(len(times)):
+        if k % 1000 == 0:
+            log(f"[bench] {tag}: request {k} at t={srv.clock() - t0:.2f} s")
+        slot = k % ring
+        if slots[slot] is not None:
+            kk, tk = slots[slot]
+            lat[kk] = srv.wait(tk)
+            if on_done:
+                on_done(kk, outs[slot])
+        arrival = t0 + float(times[k])
+        delay = arrival - srv.clock()
+        if delay > 0:
+            time.sleep(delay)
+        i, n, o = int(mesh_ids[k]), int(nq[k]), int(offs[k])
+        x, y, z = (p.array[o:o + n] for p in pools[i])
+        tk = srv.submit(i, S.MODE_SDF, x, y, z, outs[slot][0].array[:n], outs[slot][1].array[:n],
+                        flags=S.FLAG_FIXUP, arrival=arrival)
+        slots[slot] = (k, tk)
+    log(f"[bench] {tag}: all submitted at t={srv.clock() - t0:.2f} s, draining")
+    for s in range(ring):
+        if slots[s] is not None:
+            kk, tk = slots[s]
+            lat[kk] = srv.wait(tk)
+            if on_done:
+                on_done(kk, outs[s])
+    return lat
+
+
+def _window_stats(times, nq, lat, window):
+    done = times + lat
+    pend = (times <= window) & (done > window)
+    return {"p50_ms": _rnd(np.percentile(lat, 50) * 1e3, 2), "p99_ms": _rnd(np.percentile(lat, 99) * 1e3, 2),
+            "max_ms": _rnd(lat.max() * 1e3, 2), "backlog_requests_at_window_end": int(pend.sum()),
+            "backlog_queries_at_window_end": int(nq[pend].sum()),
+            "drain_s": _rnd(max(0.0, float(done.max()) - window), 3),
+            "throughput_mqps": _rnd(nq.sum() / max(float(done.max()), 1e-9) / 1e6, 2)}
+
+
+def run_serve(args, samples):
+    """Open-loop serving: Poisson arrivals over a pool of 32 resident meshes (5 k - 2 M triangles),
+    1e4 - 1e7 signed queries per request (half uniform in the padded bbox, half within 0.02 of the
+    surface), FP32 + fixup; every request takes its own points (a contiguous slice of its mesh's
+    pool of distinct points at a seeded offset); latency from the scheduled arrival to the results in
+    host memory."""
     import ssdf_amd as S
+    from ssdf_amd import meshgen as mg
     from ssdf_amd import workloads as W
     t_gen = time.perf_counter()
     meshes = W.serve_pool()
     log(f"[bench] serve: mesh pool generated ({time.perf_counter() - t_gen:.1f} s)")
-    times, mesh_ids, nq = W.poisson_requests(200.0, args.serve_seconds, len(meshes))
-    nmax = int(nq.max())
-    base = 1 << 20
-    pools = []
-    for i, m in enumerate(meshes):
-        pts = W.query_cloud(m, base, seed=100 + i)
-        arrs = []
-        for v in pts:
-            pa = S.PinnedArray(nmax, np.float32)
-            reps = -(-nmax // base)
-            pa.array[:] = np.tile(v, reps)[:nmax]
-            arrs.append(pa)
-        pools.append(arrs)
+    P = args.serve_pool_points
+    times, mesh_ids, nq = W.poisson_requests(args.serve_rate, args.serve_seconds, len(meshes))
+    nq = np.minimum(nq, P)
+    offs = (mg.SplitMix64(23).uniform(len(times)) * (P - nq + 1)).astype(np.int64)
+    pools = [[S.PinnedArray(P, np.float32) for _ in range(3)] for _ in meshes]
+    with ThreadPoolExecutor(max_workers=min(HOST_THREADS_MAX, os.cpu_count() or 1)) as ex:
+        distinct = list(ex.map(lambda i: _pool_fill(meshes[i], pools[i], P, 100 + i), range(len(meshes))))
+    # a request's points are distinct if its pool's are; otherwise at most (P - distinct) repeat
+    dup = np.array([P - d for d in distinct], np.int64)
+    distinct_q = int(np.maximum(nq - dup[mesh_ids], 0).sum())
     ring = 24
+    nmax = int(nq.max())
     outs = [(S.PinnedArray(nmax, np.float32), S.PinnedArray(nmax, np.int32)) for _ in range(ring)]
     gen_s = time.perf_counter() - t_gen
-    log(f"[bench] serve: query pools and output ring pinned ({gen_s:.1f} s)")
-    budget = 32 << 30
+    log(f"[bench] serve: {len(meshes)} pools of {P} distinct points and the output ring pinned ({gen_s:.1f} s)")
+    budget = args.serve_budget_gib << 30
     srv = S.Server(budget, max_inflight=16)
     for i, m in enumerate(meshes):
         srv.add_mesh(i, m)
@@ -229,50 +353,64 @@ def run_serve(args):
                        outs[0][1].array[:1024], flags=S.FLAG_FIXUP) for i in range(len(meshes))]
     for t in warm:
         srv.wait(t)
-    slots = [None] * ring
-    lat = []
+    # results of the first request of every mesh are sampled for the FP64 check
+    first = {}
+    for k, i in enumerate(mesh_ids):
+        first.setdefault(int(i), k)
+    want = {k: i for i, k in first.items()}
+
+    def keep(k, slot):
+        if k in want:
+            i = want[k]
+            n, o = int(nq[k]), int(offs[k])
+            sel = _pick(n, args.check_serve, seed=700 + i)
+            samples.add("served", meshes[i], *(pools[i][d].array[o + sel] for d in range(3)), slot[0].array[sel],
+                        slot[1].array[sel])
+
     t0 = srv.clock() + 0.05
     log(f"[bench] serve: {len(meshes)} meshes resident, {len(times)} requests over {args.serve_seconds} s")
-    for k in range(len(times)):
-        if k % 250 == 0:
-            log(f"[bench] serve: request {k} at t={srv.clock() - t0:.2f} s")
-        slot = k % ring
-        if slots[slot] is not None:
-            lat.append(srv.wait(slots[slot]))
-            slots[slot] = None
-        arrival = t0 + float(times[k])
-        delay = arrival - srv.clock()
-        if delay > 0:
-            time.sleep(delay)
-        i, n = int(mesh_ids[k]), int(nq[k])
-        x, y, z = (p.array[:n] for p in pools[i])
-        slots[slot] = srv.submit(i, S.MODE_SDF, x, y, z, outs[slot][0].array[:n], outs[slot][1].array[:n],
-                                 flags=S.FLAG_FIXUP, arrival=arrival)
-    log(f"[bench] serve: all submitted at t={srv.clock() - t0:.2f} s, draining")
-    for tk in slots:
-        if tk is not None:
-            lat.append(srv.wait(tk))
-    t_end = srv.clock()
+    lat = _serve_window(srv, S, meshes, pools, outs, times, mesh_ids, nq, offs, t0, on_done=keep)
     st = srv.stats()
-    srv.close()
-    lat = np.array(lat)
     total_q = int(nq.sum())
-    res = {"requests": len(times), "seconds_offered": args.serve_seconds, "meshes": len(meshes),
-           "triangles": int(sum(m.ntris for m in meshes)), "queries": total_q,
-           "offered_mqps": round(total_q / args.serve_seconds / 1e6, 2),
-           "throughput_mqps": round(total_q / (t_end - t0) / 1e6, 2),
-           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-           "max_ms": round(float(lat.max()) * 1e3, 2),
-           "mesh_loads": int(st["mesh_loads"]), "evictions": int(st["evictions"]), "errors": int(st["errors"]),
-           "hbm_budget_gib": budget / 2**30, "peak_gib": round(st["peak_bytes"] / 2**30, 3),
-           "setup_s": round(gen_s, 1)}
+    res = {"requests": len(times), "rate_req_s": args.serve_rate, "seconds_offered": args.serve_seconds,
+           "meshes": len(meshes), "triangles": int(sum(m.ntris for m in meshes)), "queries": total_q,
+           "pool_points_per_mesh": P, "distinct_fraction": _rnd(distinct_q / total_q, 6),
+           "offered_mqps": _rnd(total_q / args.serve_seconds / 1e6, 2)}
+    res.update(_window_stats(times, nq, lat, args.serve_seconds))
+    res.update({"mesh_loads": int(st["mesh_loads"]), "evictions": int(st["evictions"]), "errors": int(st["errors"]),
+                "hbm_budget_gib": budget / 2**30, "peak_gib": _rnd(st["peak_bytes"] / 2**30, 3),
+                "setup_s": _rnd(gen_s, 1)})
     log(f"[bench] serve: {res}")
+    # higher arrival rates, until one is not served with bounded latency (the backlog drains within
+    # 1 s of the window's end and p99 stays under 1 s); the highest rate that is, main run included
+    bounded = lambda w: w["drain_s"] <= 1.0 and w["p99_ms"] < 1000.0  # noqa: E731
+    best = {"rate_req_s": args.serve_rate, "offered_mqps": res["offered_mqps"], "p50_ms": res["p50_ms"],
+            "p99_ms": res["p99_ms"], "drain_s": res["drain_s"]} if bounded(res) else None
+    if args.sweep_seconds > 0 and best is not None:
+        sweep = []
+        for f in (1.25, 1.5, 1.75, 2.0, 2.5):
+            rate = f * args.serve_rate
+            tt, mi, qq = W.poisson_requests(rate, args.sweep_seconds, len(meshes), seed=40 + int(100 * f))
+            qq = np.minimum(qq, P)
+            oo = (mg.SplitMix64(41 + int(100 * f)).uniform(len(tt)) * (P - qq + 1)).astype(np.int64)
+            t1 = srv.clock() + 0.05
+            lt = _serve_window(srv, S, meshes, pools, outs, tt, mi, qq, oo, t1, tag=f"serve sweep {rate:.0f}/s")
+            w = {"rate_req_s": _rnd(rate, 1), "seconds": args.sweep_seconds,
+                 "offered_mqps": _rnd(qq.sum() / args.sweep_seconds / 1e6, 2)}
+            w.update(_window_stats(tt, qq, lt, args.sweep_seconds))
+            sweep.append(w)
+            log(f"[bench] serve sweep: {w}")
+            if not bounded(w):
+                break
+            best = w
+        res["sweep"] = sweep
+    res["max_sustained"] = best
+    srv.close()
     return res
 
 
-def run_beams(args):
-    """200 capped cylinders in the unit cube (3.4 M triangles), intact and with 1% of the triangles
+def run_beams(args, samples):
+    """200 capped cylinders in the unit cube (4.99 M triangles), intact and with 1% of the triangles
     deleted; signed distance + triangle on the 512^3 grid over the cube (FP32 + fixup; the holed
     mesh signed by winding numbers through SSDF_FLAG_SIGN_AUTO) and at 1e8 uniform scattered points."""
     import ssdf_amd as S
@@ -296,6 +434,8 @@ def run_beams(args):
         st.synchronize()
         return ev[0].elapsed_ms(ev[1])
 
+    idx = _pick(N, args.check_beams, seed=31)
+    gx, gy, gz = _grid_coords(idx, (g, g, g), (0, 0, 0), (1, 1, 1))
     outs = {}
     for key, m, flags in (("closed", full, S.FLAG_FIXUP), ("holed_gwn", holed, S.FLAG_FIXUP | S.FLAG_SIGN_AUTO)):
         rm = S.ResidentMesh.from_mesh(m, stream=st)
@@ -306,18 +446,27 @@ def run_beams(args):
         res[key] = {"grid_ms": round(ms, 2), "grid_mqps": round(N / ms / 1e3, 1), "fixup_queued": q,
                     "kernel_error": e, "open_edges": rm.info()["open_edges"]}
         outs[key] = o.download()
+        ot = t.download()
+        if key == "closed":
+            samples.add("beams200_closed", m, gx, gy, gz, outs[key][idx], ot[idx])
+        else:
+            # distances and triangles against the holed mesh, signs against the intact mesh beyond 0.01
+            samples.add("beams200_holed_gwn", m, gx, gy, gz, outs[key][idx], ot[idx], sign_mesh=full,
+                        sign_band=0.01)
         log(f"[bench] beams {key}: grid {ms:.1f} ms")
         if key == "closed":
             n = args.beams_points
-            P = np.random.default_rng(5).random((3, n), dtype=np.float32)
-            dx, dy, dz = (S.DeviceArray.from_host(P[d]) for d in range(3))
-            del P
+            rng = np.random.default_rng(5)
+            dx, dy, dz = (S.DeviceArray.from_host(rng.random(n, dtype=np.float32)) for _ in range(3))
             po = S.DeviceArray(n, np.float32)
             pt = S.DeviceArray(n, np.int32)
             ms = timed(lambda: rm.query_points(S.MODE_SDF, dx, dy, dz, po, pt, flags=flags, stream=st))
             q, e = rm.last_stats()
             res["scattered"] = {"points": n, "ms": round(ms, 2), "mqps": round(n / ms / 1e3, 1), "fixup_queued": q,
                                 "kernel_error": e}
+            sel = _pick(n, args.check_beams, seed=77)
+            samples.add("beams200_scattered", m, dx.download()[sel], dy.download()[sel], dz.download()[sel],
+                        po.download()[sel], pt.download()[sel])
             log(f"[bench] beams scattered: {ms:.1f} ms")
             del dx, dy, dz, po, pt
         del rm, o, t
@@ -327,25 +476,116 @@ def run_beams(args):
     return res
 
 
-def cpu_baseline(args, m, n, lo, hi):
-    """The oracle's edf_celllist<float> restatement (serial/ssdf.hpp:642-963) on a seeded uniform
-    subsample of the grid, all host threads."""
+def host_cores():
+    """(logical CPUs this process may run on, physical cores among them)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((base.joinpath("physical_package_id").read_text().strip(),
+                       base.joinpath("core_id").read_text().strip()))
+        except OSError:
+            cores.add(("cpu", str(c)))
+    return len(cpus), len(cores)
+
+
+def cpu_baseline(args, m, n, lo, hi, samples, legs):
+    """The cpu_baseline leg.  Timing: the oracle's restatement of edf_celllist<float>
+    (serial/ssdf.hpp:642-963, its static `omp parallel for` schedule) on a seeded uniform subsample
+    of the grid, one thread per physical core (at most the box's share), OMP_PROC_BIND=close,
+    OMP_PLACES=cores.  Checking: every leg's sampled results against the oracle's exact FP64 argmin
+    (its CPU BVH) and the reference's pseudonormal sign rule (bvh.cpp:1734-1792)."""
+    nlog, nphys = host_cores()
+    threads = max(1, min(nphys, HOST_THREADS_MAX))
+    # the OpenMP runtime reads these when the oracle library loads (nothing has loaded it yet)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
     from oracle import oracle as O
     from ssdf_amd import meshgen as mg
+    O._load()
     rng = mg.SplitMix64(12345)
     N = n[0] * n[1] * n[2]
-    k = args.cpu_sample
-    idx = np.unique((rng.uniform(k) * N).astype(np.int64))
-    ax = [mg.grid_axis(n[d], lo[d], hi[d]) for d in range(3)]
-    i, j, kk = idx % n[0], (idx // n[0]) % n[1], idx // (n[0] * n[1])
-    x, y, z = ax[0][i], ax[1][j], ax[2][kk]
+    idx = np.unique((rng.uniform(args.cpu_sample) * N).astype(np.int64))
+    x, y, z = _grid_coords(idx, n, lo, hi)
     out = np.full(idx.size, np.finfo(np.float32).max, np.float32)
     t0 = time.perf_counter()
     O.celllist(x, y, z, m, out)
     dt = time.perf_counter() - t0
-    return {"value": idx.size / dt / 1e6, "unit": "Mqueries/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{idx.size} grid nodes drawn uniformly (SplitMix64 seed 12345) from the 256^3 grid; "
-                      f"edf_celllist<float> restated in C, OpenMP, {dt:.2f} s incl. its cell build"}
+    res = {"value": idx.size / dt / 1e6, "unit": "Mqueries/s", "cores": threads, "kind": "port",
+           "threads": O.num_threads(), "host_logical_cpus": nlog, "host_physical_cores": nphys,
+           "sample": f"{idx.size} grid nodes drawn uniformly (SplitMix64 seed 12345) from the 256^3 grid; "
+                     f"edf_celllist<float> restated in C (static OpenMP schedule as ssdf.hpp:912), "
+                     f"{O.num_threads()} threads on {threads} physical cores (OMP_PLACES=cores, "
+                     f"OMP_PROC_BIND=close), {dt:.2f} s incl. its cell build"}
+    log(f"[bench] cpu_baseline: {res['value']:.3f} Mq/s on {threads} cores ({dt:.1f} s)")
+    # ---- accuracy of every leg against the FP64 oracle ----
+    t0 = time.perf_counter()
+    by_mesh = {}
+    for it in samples.items:
+        by_mesh.setdefault(id(it["mesh"]), []).append(it)
+    acc = {}
+    normals = {}
+
+    def normals_of(mesh):
+        if id(mesh) not in normals:
+            normals[id(mesh)] = O.pseudonormals(mesh)
+        return normals[id(mesh)]
+
+    for items in by_mesh.values():
+        mesh = items[0]["mesh"]
+        X, Y, Z = (np.concatenate([it[c] for it in items]) for c in "xyz")
+        d2all, rtri = O.bvh_argmin(X, Y, Z, mesh)
+        sg, _, _ = O.sign(X, Y, Z, rtri, mesh, normals_of(mesh))
+        s = sg * np.sqrt(d2all)
+        lo_m, hi_m = mesh.bbox()
+        band = 1e-6 * float(np.linalg.norm(hi_m - lo_m))
+        k = 0
+        for it in items:
+            nn = it["x"].size
+            ref = s[k:k + nn]
+            rt = rtri[k:k + nn]
+            d2r = d2all[k:k + nn]
+            k += nn
+            if it["sign_mesh"] is not None:
+                sref, _, _, _, _ = O.sdf(it["x"], it["y"], it["z"], it["sign_mesh"],
+                                         normals=normals_of(it["sign_mesh"]))
+                sband = it["sign_band"]
+            else:
+                sref, sband = ref, band
+            d_ref = np.abs(ref)
+            got = np.abs(it["d"].astype(np.float64))
+            e = got - d_ref
+            rel = np.abs(e) / np.maximum(d_ref, 1e-30)
+            far = np.abs(sref) > sband
+            a = acc.setdefault(it["leg"], {"n": 0, "max_err": 0.0, "sq": 0.0, "max_rel_err": 0.0,
+                                           "index_mismatch": 0, "index_mismatch_not_tied": 0,
+                                           "sign_mismatch_outside_band": 0,
+                                           "sign_band": "1e-6*bbox" if it["sign_mesh"] is None else
+                                           f"{sband} (intact mesh's sign)"})
+            a["n"] += nn
+            a["max_err"] = max(a["max_err"], float(np.abs(e).max()))
+            a["sq"] += float((e * e).sum())
+            a["max_rel_err"] = max(a["max_rel_err"], float(rel.max()))
+            mis = it["tri"] != rt
+            a["index_mismatch"] += int(mis.sum())
+            if mis.any():
+                # a differing triangle at exactly the argmin's FP64 distance is a tie, which the plain
+                # FP32 path may break differently (FP32 + fixup takes the lowest index, as the oracle)
+                d2_got = O.pair_d2(it["x"][mis], it["y"][mis], it["z"][mis], it["tri"][mis], mesh)
+                a["index_mismatch_not_tied"] += int(np.sum(d2_got != d2r[mis]))
+            a["sign_mismatch_outside_band"] += int(np.sum((np.sign(it["d"]) != np.sign(sref)) & far))
+    for leg, a in acc.items():
+        a["rms_err"] = math.sqrt(a.pop("sq") / max(a["n"], 1))
+        a["max_err"] = float(f"{a['max_err']:.4g}")
+        a["rms_err"] = float(f"{a['rms_err']:.4g}")
+        a["max_rel_err"] = float(f"{a['max_rel_err']:.4g}")
+        a["reference"] = "oracle FP64 argmin (CPU BVH) + pseudonormal sign"
+        legs.setdefault(leg, a)
+    log(f"[bench] FP64 checks of {sum(a['n'] for a in acc.values())} sampled results in "
+        f"{time.perf_counter() - t0:.1f} s")
+    return res
 
 
 def measured_traffic(kernel):
@@ -403,7 +643,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=1 << 23)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg (and the FP64 checks)")
     ap.add_argument("--only", choices=["fixup", "plain", "batch"], default=None,
                     help="time one mode (or the 64-mesh batch leg) only and print nothing else (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-mesh batch leg")
@@ -411,7 +651,15 @@ def main():
     ap.add_argument("--no-beams", action="store_true", help="skip the 200-beam stress leg")
     ap.add_argument("--beams-grid", type=int, default=512)
     ap.add_argument("--beams-points", type=int, default=100_000_000)
-    ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-pool-points", type=int, default=10_000_000, help="distinct points per mesh pool")
+    ap.add_argument("--serve-budget-gib", type=int, default=64, help="HBM budget of the server")
+    ap.add_argument("--sweep-seconds", type=float, default=8.0, help="window of each lower-rate run (0: none)")
+    ap.add_argument("--check-c2", type=int, default=1 << 18, help="sampled nodes checked against FP64")
+    ap.add_argument("--check-batch", type=int, default=1024, help="sampled nodes per batch mesh")
+    ap.add_argument("--check-serve", type=int, default=512, help="sampled points per mesh (served)")
+    ap.add_argument("--check-beams", type=int, default=1 << 16, help="sampled queries per beam query set")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -419,13 +667,14 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (queries shard by "
               f"replicating the mesh and splitting the queries)", file=sys.stderr)
         sys.exit(2)
+    samples = Samples()
     if args.only == "batch":
-        log(json.dumps(run_batch(args)))
+        log(json.dumps(run_batch(args, samples)))
         return
     m, n, lo, hi = make_workload(args)
     N = n[0] * n[1] * n[2]
     log(f"[bench] mesh {m.name}: {m.ntris} triangles; grid {n}; steps {args.steps} warmup {args.warmup}")
-    g = run_gpu(args, m, n, lo, hi)
+    g = run_gpu(args, m, n, lo, hi, samples)
     if args.copy_calibration:
         import ssdf_amd as S
         nbytes = args.copy_calibration << 20
@@ -436,29 +685,43 @@ def main():
     if args.only:
         log(json.dumps(g))
         return
+    errs = {}
     line = {
         "metric": METRIC, "value": round(g["fixup"]["mqps"], 2), "unit": "Mqueries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(g["fixup"]["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"noisy torus 200k tris x {n[0]}^3 grid, signed distance + closest triangle, "
-                               "FP32 + FP64 fixup; step = in-place LBVH + pseudonormal rebuild + query",
+                               "FP32 + FP64 fixup; step = in-place LBVH + pseudonormal rebuild + query, inputs and "
+                               "outputs resident in HBM (host-to-host rates: host_call_*)",
                    "triangles": m.ntris, "queries": N, "mesh_checksum": m.checksum()},
         "plain": {"value": round(g["plain"]["mqps"], 2), "ms_per_step": round(g["plain"]["ms_per_step"], 4),
                   "query_ms": round(g["plain"]["query_ms"], 4)},
         "fixup_query_only": {"value": round(g["fixup"]["query_mqps"], 2), "query_ms": round(g["fixup"]["query_ms"], 4)},
         "host_call_plain": {"value": round(g["host_call_plain_mqps"], 2), "seconds": round(g["host_call_plain_s"], 4)},
+        "host_call_fixup": {"value": round(g["host_call_fixup_mqps"], 2), "seconds": round(g["host_call_fixup_s"], 4)},
         "fixup_queued": g["fixup_queued"], "kernel_error": g["kernel_error"], "lbvh_maxdepth": g["maxdepth"],
         "build_ms": round(g["build_ms"], 3),
     }
     if args.grid == 256:
         line["roofline"] = roofline(g["fixup"]["query_ms"], N, m.nverts, m.ntris)
     if not args.no_batch:
-        line["batch64"] = run_batch(args)
-    line["served"] = None if args.no_serve else run_serve(args)
+        line["batch64"] = run_batch(args, samples)
+    line["served"] = None if args.no_serve else run_serve(args, samples)
     if not args.no_beams:
-        line["beams200"] = run_beams(args)
+        line["beams200"] = run_beams(args, samples)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi)
+        line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi, samples, errs)
+        line["error_vs_fp64"] = errs
+        for key, leg in (("fixup_query_only", "c2_fixup"), ("plain", "c2_plain")):
+            line[key]["error_vs_fp64"] = errs.get(leg)
+        if line.get("batch64"):
+            line["batch64"]["error_vs_fp64"] = errs.get("batch64")
+        if line.get("served"):
+            line["served"]["error_vs_fp64"] = errs.get("served")
+        if line.get("beams200"):
+            line["beams200"]["closed"]["error_vs_fp64"] = errs.get("beams200_closed")
+            line["beams200"]["scattered"]["error_vs_fp64"] = errs.get("beams200_scattered")
+            line["beams200"]["holed_gwn"]["error_vs_fp64"] = errs.get("beams200_holed_gwn")
     print(json.dumps(line), flush=True)
 
 

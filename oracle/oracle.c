@@ -92,6 +92,22 @@ void orc_brute_argmin(ptrdiff_t npoints, const double *x, const double *y, const
     }
 }
 
+void orc_pair_d2(ptrdiff_t npoints, const double *x, const double *y, const double *z, const int *tri,
+                 const int *s0, const int *s1, const int *s2, const double *sx, const double *sy, const double *sz,
+                 double *out_d2) {
+#pragma omp parallel for
+    for (ptrdiff_t p = 0; p < npoints; p++) {
+        const int t = tri[p];
+        if (t < 0) {
+            out_d2[p] = INFINITY;
+            continue;
+        }
+        const int i0 = s0[t], i1 = s1[t], i2 = s2[t];
+        out_d2[p] = pt_tri_d2_d(x[p], y[p], z[p], sx[i0], sy[i0], sz[i0], sx[i1], sy[i1], sz[i1], sx[i2], sy[i2],
+                                sz[i2]);
+    }
+}
+
 /* ================================================================================================
  * LBVH topology (Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees and k-d
  * trees"): shared by the exact FP64 BVH oracle and the canonical FP32 work-count replay.

@@ -67,6 +67,11 @@ void orc_brute_argmin(ptrdiff_t npoints, const double *x, const double *y, const
 
 /* The same argmin (identical results) through a CPU BVH with a conservative pruning margin, for
  * inputs where brute force is too slow. */
+/* FP64 squared distance of point i to triangle tri[i] (pt_tri_d2, serial/ssdf.hpp:50-121), for
+ * judging a reported triangle against the argmin: equal d2 means an exact tie. */
+void orc_pair_d2(ptrdiff_t npoints, const double *x, const double *y, const double *z, const int *tri,
+                 const int *s0, const int *s1, const int *s2, const double *sx, const double *sy, const double *sz,
+                 double *out_d2);
 void orc_bvh_argmin(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nsel,
                     const int *s0, const int *s1, const int *s2, ptrdiff_t nsp, const double *sx, const double *sy,
                     const double *sz, double *out_d2, int *out_tri);

@@ -40,6 +40,7 @@ def _load():
             "orc_celllist_f": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P], C.c_int),
             "orc_celllist_d": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P], C.c_int),
             "orc_brute_argmin": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+            "orc_pair_d2": ([_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_bvh_argmin": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P], None),
             "orc_pseudonormals": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P], _PD),
             "orc_sign": ([_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
@@ -184,6 +185,18 @@ def brute_argmin(x, y, z, mesh, init_d2=None):
     L.orc_brute_argmin(n, _p(x), _p(y), _p(z), s0.size, _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), _p(init),
                        _p(d2), _p(tri), _p(ntie), _p(sec))
     return d2, tri, ntie, sec
+
+
+def pair_d2(x, y, z, tri, mesh):
+    """FP64 squared distance of each point to its given triangle (-1: +inf)."""
+    L = _load()
+    x, y, z = _f64(x, y, z)
+    sx, sy, sz = _f64(mesh.sx, mesh.sy, mesh.sz)
+    s0, s1, s2 = _i32(mesh.s0, mesh.s1, mesh.s2)
+    t = np.ascontiguousarray(tri, dtype=np.int32)
+    out = np.empty(x.size)
+    L.orc_pair_d2(x.size, _p(x), _p(y), _p(z), _p(t), _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), _p(out))
+    return out
 
 
 def bvh_argmin(x, y, z, mesh):

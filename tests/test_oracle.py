@@ -279,6 +279,18 @@ def test_brute_argmin_exact_small(oracle):
         assert tri[p] == int(np.argmin([float(e) for e in ex]))
 
 
+def test_pair_d2_is_the_argmin_distance(oracle):
+    """pair_d2 at the argmin's own triangle reproduces the argmin's d2 bit for bit; at another
+    triangle it is never below it (the argmin is a minimum over the same function)."""
+    m = mg.torus(0.6, 0.2, 60, 24, sigma=0.005, seed=7)
+    x, y, z = mg.grid_points(12, 12, 12, (-1, -1, -1), (1, 1, 1))
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.array_equal(oracle.pair_d2(x, y, z, tri, m), d2)
+    other = (tri + 17) % m.ntris
+    assert np.all(oracle.pair_d2(x, y, z, other, m) >= d2)
+    assert np.all(np.isinf(oracle.pair_d2(x[:3], y[:3], z[:3], np.full(3, -1, np.int32), m)))
+
+
 @pytest.mark.parametrize("meshname", ["sphere", "torus_small", "soup"])
 def test_bvh_argmin_equals_brute(oracle, meshname):
     if meshname == "sphere":
