@@ -249,8 +249,12 @@ __host__ __device__ __forceinline__ float tri_d2_fast(const float px, const floa
 }
 
 // Fills a TriP from FP32 vertices: the constants are evaluated in FP64 from the FP32 values and
-// rounded once.  A triangle whose normal vanishes against its edges (|n|^2 <= 1e-24 |e|^4) is
-// degenerate: NaN side normals send every query to the segment branch of tri_d2_fast.
+// rounded once.  A sliver whose smallest height is at most 1e-6 of its longest edge
+// (|n| <= 1e-6 |e|^2, n = 2 x area) is flagged degenerate: NaN side normals send every query to the
+// segment branch of tri_d2_fast, whose error is then at most that height (a projection inside the
+// triangle lies within it of an edge).  Thicker triangles meet the FP32 error bound of the fixup
+// band (2^-18 of the coordinate scale); thinner unflagged ones do not: their FP32 side tests cannot
+// resolve the strip they cover, and points along their axis were taken as inside.
 __host__ __device__ inline void make_trip(TriP &tr, const float *va, const float *vb, const float *vc, int orig) {
     double v[3][3];
     for (int d = 0; d < 3; d++) {
@@ -270,7 +274,7 @@ __host__ __device__ inline void make_trip(TriP &tr, const float *va, const float
         const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
         emax = l2 > emax ? l2 : emax;
     }
-    const bool degenerate = !(emax > 0) || !(nn > 1e-24 * emax * emax);
+    const bool degenerate = !(emax > 0) || !(nn > 1e-12 * emax * emax);
     const double inv_nl = degenerate ? 0.0 : 1.0 / sqrt(nn);
     for (int q = 0; q < 3; q++) {
         const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];

@@ -399,7 +399,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             // the bound handed over is the FP32 band of the best so far (plain mode too: the
             // per-query walk then finds that triangle again even where an ancestor box's FP32
             // distance rounds above the triangle's)
-            a.fix_list[4 * slot + 1] = __float_as_uint(band_sq(FIX ? bestd : best, tau1, babs));
+            // (FIX with a sliver flagged degenerate as the best so far: no bound, since the FP64
+            // routine's value for such a triangle can exceed its FP32 one by more than the band)
+            const bool sliver = FIX && bk >= 0 && (tris[bk].f.flags & 1);
+            a.fix_list[4 * slot + 1] = __float_as_uint(sliver ? SSDF_INF : band_sq(FIX ? bestd : best, tau1, babs));
             a.fix_list[4 * slot + 2] = a.fix_tag;
             a.fix_list[4 * slot + 3] = (unsigned)((unsigned long long)qid >> 32);
             active = false;
@@ -417,10 +420,16 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     int k_final = bk;      // leaf position
     bool have64 = false;   // distance known in FP64
     double d2_64 = 0.0;
+    bool unbounded = false;  // re-search in FP64 without the FP32 band
     if (FIX && bk >= 0) {
         bool exact_needed = ncand > kCand;
         if (!exact_needed && (ncand >= 2 || (tris[bk].f.flags & 1))) {
             const double px = qx, py = qy, pz = qz;
+            // On slivers flagged degenerate (height below 1e-6 of the longest edge) the FP64
+            // routine's value can exceed the true distance, which the FP32 one follows, by more than
+            // the band: the list then need not hold the FP64 argmin, so such a query is searched
+            // again in FP64 without a bound.
+            const double band64 = (double)band_sq(bestd, tau1, babs);
             double bestd = __builtin_inf();
             int bestt = INT_MAX, bestk = -1;
             for (int i = 0; i < ncand; i++) {
@@ -428,9 +437,12 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
                 const TriF tr = tris[kc].f;
                 const double d = pt_tri_d2<double>(px, py, pz, tr.a[0], tr.a[1], tr.a[2], tr.b[0], tr.b[1], tr.b[2],
                                                    tr.c[0], tr.c[1], tr.c[2]);
+                if ((tr.flags & 1) && d > band64) unbounded = true;
                 if (d < bestd || (d == bestd && tr.orig < bestt)) bestd = d, bestt = tr.orig, bestk = kc;
             }
-            if (bestk >= 0) {
+            if (unbounded) {
+                exact_needed = true;
+            } else if (bestk >= 0) {
                 k_final = bestk;
                 have64 = true;
                 d2_64 = bestd;
@@ -451,7 +463,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             const unsigned pos = atomicAdd(a.fix_count, 1u);
             if (pos < a.fix_cap) {
                 a.fix_list[4 * pos] = (unsigned)qid;
-                a.fix_list[4 * pos + 1] = __float_as_uint(band_sq(bestd, tau1, babs));
+                a.fix_list[4 * pos + 1] = __float_as_uint(unbounded ? SSDF_INF : band_sq(bestd, tau1, babs));
                 a.fix_list[4 * pos + 2] = a.fix_tag;
                 a.fix_list[4 * pos + 3] = (unsigned)((unsigned long long)qid >> 32);
                 return;

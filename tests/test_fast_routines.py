@@ -2,7 +2,8 @@
 built for the host from the same header the kernels use, against the FP64 oracle: the triangle
 distance agrees to the FP32 error bound the fixup band assumes (2^-18 of the coordinate scale),
 the box distance never exceeds the triangle distance by more than that bound (pruning stays
-conservative), and degenerate triangles get their segment distance."""
+conservative), degenerate triangles get their segment distance, and slivers of every thickness
+stay within the bound of their exact distance."""
 import subprocess
 from pathlib import Path
 
@@ -42,8 +43,8 @@ def test_tri_d2_fast_accuracy(exe, oracle):
     ref = np.array([oracle.pt_tri_d2(*(np.float64(v) for v in cs)) for cs in cases])
     L = np.array([max(1.0, float(np.abs(np.concatenate(cs)).max())) for cs in cases])
     err = np.abs(np.sqrt(d) - np.sqrt(ref))
-    assert np.all(err <= 2.0 ** -18 * L), err.max()
-    assert np.all(np.sqrt(bx) <= np.sqrt(ref) + 2.0 ** -18 * L)  # conservative pruning
+    assert np.all(err <= 2.0 ** -19 * L), err.max()  # half the fixup band's absolute term
+    assert np.all(np.sqrt(bx) <= np.sqrt(ref) + 2.0 ** -19 * L)  # conservative pruning
     assert np.all(fl == 0)
 
 
@@ -67,3 +68,86 @@ def test_tri_d2_fast_degenerate(exe, oracle):
             t = 0.0 if e @ e == 0 else np.clip((p - u) @ e / (e @ e), 0, 1)
             seg.append(np.sum((p - u - t * e) ** 2))
         assert abs(np.sqrt(dd) - np.sqrt(min(seg))) <= 2.0 ** -18 * 2
+
+
+def _sliver_cases(seed, per_shape=6):
+    """Slivers of height/length 1e-2 down to 1e-11, as needles (apex near a vertex) and caps (apex
+    near the middle of the long edge), randomly oriented, scaled and placed; query points in all
+    seven Voronoi regions (the three vertices, the three edges, the face) at several heights above
+    the plane.  52 points per triangle, 2 * per_shape triangles per aspect ratio."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(2, 12):
+        aspect = 10.0 ** -k
+        for shape in ("needle", "cap"):
+            for _ in range(per_shape):
+                q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+                length = 10.0 ** rng.uniform(-3, 0)
+                origin = rng.uniform(-1, 1, 3)
+                u = 0.02 * rng.uniform() if shape == "needle" else 0.5 + 0.1 * rng.uniform(-1, 1)
+                local = np.array([[0, 0, 0], [1, 0, 0], [u, aspect, 0]]) * length
+                a, b, c = (origin + q @ v for v in local)
+                a, b, c = (v.astype(np.float32) for v in (a, b, c))
+                af, bf, cf = (v.astype(np.float64) for v in (a, b, c))
+                n = np.cross(bf - af, cf - af)
+                n /= np.linalg.norm(n)
+                cen = (af + bf + cf) / 3
+                pts = []
+                for h in (0.0, 1e-6, 1e-3, 0.3):
+                    hn = h * length * n
+                    pts.append(cen + hn)                                     # face
+                    for v in (af, bf, cf):                                   # vertices, outward
+                        pts.append(v + 0.3 * length * (v - cen) / np.linalg.norm(v - cen) + hn)
+                    for v0, v1 in ((af, bf), (bf, cf), (cf, af)):            # edges, outward in plane
+                        mid = 0.5 * (v0 + v1)
+                        out = np.cross(v1 - v0, n)
+                        out /= np.linalg.norm(out)
+                        if out @ (mid - cen) < 0:
+                            out = -out
+                        for s in (1e-7, 1e-4, 0.1):
+                            pts.append(mid + s * length * out + hn)
+                for p in pts:
+                    cases.append((p.astype(np.float32), a, b, c))
+    return cases
+
+
+def test_tri_d2_fast_slivers(exe, oracle):
+    """The fixup band's absolute term (2^-18 of the coordinate scale L) is twice the FP32 routine's
+    error bound, |sqrt(d_fast) - sqrt(d)| <= 2^-19 L, here against the EXACT distance (rational
+    arithmetic) on slivers of every thickness: those thinner than 1e-6 (height / longest edge) are
+    flagged degenerate and measured by their edge segments, the others through the side tests.
+
+    The reference's FP64 routine (serial/ssdf.hpp:50-121, the fixup's re-evaluation) is itself
+    inaccurate on the flagged slivers: its face-region barycentrics cancel, and it overestimates the
+    distance by up to about 1e-4 here.  The FP32 routine meets the bound against it only where it
+    is accurate (unflagged triangles); the fixup re-searches a query in FP64 without a bound when a
+    flagged candidate's FP64 value leaves the band."""
+    from tests.test_oracle import exact_tri_d2
+    per_shape = 6
+    cases = _sliver_cases(31, per_shape)
+    d, bx, fl = run(exe, cases)
+    ericson = np.array([oracle.pt_tri_d2(*(np.float64(v) for v in cs)) for cs in cases])
+    exact = np.array([float(exact_tri_d2(*cs)) for cs in cases])
+    L = np.array([max(1.0, float(np.abs(np.concatenate(cs)).max())) for cs in cases])
+    err = np.abs(np.sqrt(d) - np.sqrt(exact))
+    assert np.all(err <= 2.0 ** -19 * L), (err / L).max()
+    assert np.all(np.sqrt(bx) <= np.sqrt(exact) + 2.0 ** -19 * L)  # conservative pruning
+    unflagged = fl == 0
+    err64 = np.abs(np.sqrt(d) - np.sqrt(ericson))
+    assert np.all(err64[unflagged] <= 2.0 ** -19 * L[unflagged]), (err64[unflagged] / L[unflagged]).max()
+    # flagged exactly when the FP32 triangle's height is below 1e-6 of its longest edge (the nominal
+    # aspect of the thinner slivers is lost to the rounding of their vertices to FP32)
+    h_rel = []
+    for p, a, b, c in cases:
+        a, b, c = (v.astype(np.float64) for v in (a, b, c))
+        n = np.linalg.norm(np.cross(b - a, c - a))
+        e = max(np.sum((b - a) ** 2), np.sum((c - b) ** 2), np.sum((a - c) ** 2))
+        h_rel.append(n / e)
+    h_rel = np.array(h_rel)
+    clear = np.abs(np.log(h_rel / 1e-6)) > 0.01
+    assert np.array_equal(fl[clear] == 1, h_rel[clear] < 1e-6)
+    assert 0 < int(fl.sum()) < fl.size
+    over = np.sqrt(ericson) - np.sqrt(exact)
+    assert over.min() >= -2.0 ** -30  # the FP64 routine never undershoots the exact distance here
+    print(f"slivers: {len(cases)} cases, {int(fl.sum())} flagged, worst FP32 error {float((err / L).max()):.3g} L; "
+          f"FP64 routine above the exact distance by up to {float(over.max()):.3g}")

@@ -442,3 +442,77 @@ def test_sample_sqedf(oracle):
     assert np.all(np.abs(outf.astype(np.float64) - want) <= bars.REL * want + bars.FLOOR)
     stay = (seed < 1) & (d2 > 0.0011)  # seeds clearly below d^2 are kept bit for bit
     assert stay.sum() > 100 and np.all(outf[stay] == np.float32(0.001))
+
+
+def _seg_d(p, u, v):
+    e = v - u
+    t = np.clip((p - u) @ e / (e @ e), 0.0, 1.0)
+    return float(np.linalg.norm(p - u - t * e))
+
+
+def _sliver_traps(oracle, ntraps, seed):
+    """A mesh of `ntraps` spread-out traps and the query points that spring them.  A trap is a
+    needle-shaped sliver (0.1 to 0.5 long, height 1e-8 to 1e-10 of that, flagged degenerate) on whose face the
+    reference's FP64 routine overestimates the distance of a point p by more than five times the
+    fixup band, plus a small triangle at a distance from p between the sliver's true distance (that
+    of its nearest edge, to within its height) and that FP64 value: the FP64 argmin (the oracle's
+    brute force) is the small triangle, while the FP32 search finds the sliver first."""
+    rng = np.random.default_rng(seed)
+    verts, tris, pts = [], [], []
+    while len(pts) < ntraps:
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        length = 10 ** rng.uniform(-1, -0.3)
+        k = len(pts)
+        origin = np.array([k % 4, k // 4 % 4, k // 16], np.float64) - 1.5  # 1 apart: traps never meet
+        shape = np.array([[0, 0, 0], [1, 0, 0], [0.02 * rng.uniform(), 10.0 ** -rng.uniform(8, 10), 0]]) * length
+        a, b, c = (np.asarray(origin + q @ v, np.float32) for v in shape)
+        af, bf, cf = (v.astype(np.float64) for v in (a, b, c))
+        n = np.cross(bf - af, cf - af)
+        if not np.linalg.norm(n) > 0:
+            continue  # collinear once rounded to FP32
+        n /= np.linalg.norm(n)
+        p = ((af + bf + cf) / 3 + 1e-4 * length * n).astype(np.float32)
+        pf = p.astype(np.float64)
+        d64 = np.sqrt(oracle.pt_tri_d2(pf, af, bf, cf))
+        true = min(_seg_d(pf, af, bf), _seg_d(pf, bf, cf), _seg_d(pf, cf, af))
+        if not 2e-5 <= d64 - true <= 1e-3:
+            continue
+        r = 0.5 * (d64 + true)  # the small triangle's distance from p
+        dirn = n if rng.random() < 0.5 else np.cross(n, bf - af) / np.linalg.norm(bf - af)
+        centre = pf + (r + 1e-7) * dirn
+        w1 = np.cross(dirn, [1.0, 0, 0] if abs(dirn[0]) < 0.9 else [0, 1.0, 0])
+        w1 /= np.linalg.norm(w1)
+        w2 = np.cross(dirn, w1)
+        base = len(verts)
+        verts += [a, b, c]
+        verts += [np.asarray(centre + 1e-7 * w, np.float32) for w in (w1, -0.5 * w1 + 0.87 * w2, -0.5 * w1 - 0.87 * w2)]
+        tris += [(base, base + 1, base + 2), (base + 3, base + 4, base + 5)]
+        pts.append(p)
+    V = np.array(verts, np.float32)
+    F = np.array(tris, np.int32)
+    m = mg.Mesh(*(np.ascontiguousarray(V[:, k]) for k in range(3)), *(np.ascontiguousarray(F[:, k]) for k in range(3)))
+    P = np.array(pts, np.float32)
+    return m, [np.ascontiguousarray(P[:, k]) for k in range(3)]
+
+
+@pytest.mark.parametrize("leaves", [0, 1])
+def test_fixup_slivers_follow_the_fp64_routine(oracle, leaves):
+    """Where the reference's FP64 routine overestimates a sliver's distance beyond the band, the fixup
+    still returns the FP64 argmin: the query is searched again in FP64 without a bound (in the packet
+    epilogue, and for waves handed over past their leaf budget)."""
+    m, (x, y, z) = _sliver_traps(oracle, 40, seed=3)
+    d2, tri, _, _ = oracle.brute_argmin(x, y, z, m)
+    assert np.all(tri % 2 == 1)  # the small triangles, not the slivers
+    rm = S.ResidentMesh.from_mesh(m, with_normals=False)
+    dx, dy, dz = _dev(x, y, z)
+    out = S.DeviceArray.from_host(np.full(x.size, np.inf, np.float32))
+    t = S.DeviceArray.from_host(np.full(x.size, -1, np.int32))
+    prev = S.set_heavy_leaves(leaves)
+    try:
+        rm.query_points(S.MODE_EDF, dx, dy, dz, out, t, flags=S.FLAG_FIXUP)
+        q, err = rm.last_stats()
+    finally:
+        S.set_heavy_leaves(prev)
+    assert err == 0 and q >= x.size
+    assert np.array_equal(t.download(), tri)
+    bars.assert_fixup_distances(out.download(), np.sqrt(d2), "sliver traps")
