@@ -129,7 +129,7 @@ def run_gpu(args, m, n, lo, hi, samples):
     modes = (("fixup", S.FLAG_FIXUP), ("plain", 0))
     if args.only:
         modes = tuple(md for md in modes if md[0] == args.only)
-    idx = _pick(N, args.check_c2, seed=12)
+    idx = _pick(N, args.check_torus, seed=12)
     sx, sy, sz = _grid_coords(idx, n, lo, hi)
     for name, flags in modes:
         qt = []
@@ -146,7 +146,7 @@ def run_gpu(args, m, n, lo, hi, samples):
                      "mqps": N / dt / 1e6, "query_mqps": N / (np.mean(qt) * 1e-3) / 1e6}
         log(f"[bench] {name}: step {dt * 1e3:.3f} ms, query {np.mean(qt):.3f} ms")
         if not args.only:
-            samples.add(f"c2_{name}", m, sx, sy, sz, out.download()[idx], tri.download()[idx])
+            samples.add(f"torus200k_{name}", m, sx, sy, sz, out.download()[idx], tri.download()[idx])
     if args.only:
         return res
     # fixup statistics of one more fixup query on the resident mesh
@@ -246,6 +246,15 @@ def run_batch(args, samples):
         log(f"[bench] batch64 {key}: {ms:.3f} ms")
     q, e = b.last_stats()
     res["fixup_queued"], res["kernel_error"] = q, e
+    # the batch query against the FP32 vector peak: canonical per-query work of every mesh's grid
+    wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["batch64_icospheres_grids"]
+    if wc["queries"] == nq:
+        flops = FLOPS_BOX * wc["box_tests"] + FLOPS_TRI * wc["tri_tests"] + FLOPS_SIGN * nq
+        ach = flops / (res["resident_query_ms"] * 1e-3) / 1e12
+        res["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "algorithmic_gflop": round(flops / 1e9, 1),
+                           "useful_box_frac": wc["packet"]["useful_box_frac"],
+                           "useful_tri_frac": wc["packet"]["useful_tri_frac"]}
     res["speedup_resident_step_vs_per_mesh_calls"] = res["resident_step_mqps"] / res["per_mesh_calls_mqps"]
     res["speedup_host_to_host_vs_per_mesh_calls"] = res["batch_host_to_host_mqps"] / res["per_mesh_calls_mqps"]
     ho, ht = out.download(), tri.download()
@@ -616,7 +625,7 @@ def measured_valu_instructions(kernel_prefix):
 
 
 def roofline(query_ms, n_queries, nverts, ntris):
-    wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["c2_torus_200k_grid256"]
+    wc = json.loads((ROOT / "tests" / "golden" / "work_counts.json").read_text())["torus_200k_grid256"]
     assert wc["queries"] == n_queries
     flops = FLOPS_BOX * wc["box_tests"] + FLOPS_TRI * wc["tri_tests"] + FLOPS_SIGN * n_queries
     achieved = flops / (query_ms * 1e-3) / 1e12
@@ -632,6 +641,8 @@ def roofline(query_ms, n_queries, nverts, ntris):
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": measured_traffic("k_query_packet_fixup"),
             "kernel": "k_query_packet (FP32+fixup)", "algorithmic_gflop_per_launch": round(flops / 1e9, 2),
             "valu_issue_frac": issue,
+            # share of the packet walk's executed per-lane tests that the canonical per-query walk needs
+            "useful_box_frac": wc["packet"]["useful_box_frac"], "useful_tri_frac": wc["packet"]["useful_tri_frac"],
             "hbm": {"achieved": round(hbm, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hbm / PEAK_HBM_GBS, 5),
                     "algorithmic_bytes_per_launch": hbm_bytes}}
 
@@ -656,7 +667,7 @@ def main():
     ap.add_argument("--serve-pool-points", type=int, default=10_000_000, help="distinct points per mesh pool")
     ap.add_argument("--serve-budget-gib", type=int, default=64, help="HBM budget of the server")
     ap.add_argument("--sweep-seconds", type=float, default=8.0, help="window of each lower-rate run (0: none)")
-    ap.add_argument("--check-c2", type=int, default=1 << 18, help="sampled nodes checked against FP64")
+    ap.add_argument("--check-torus", type=int, default=1 << 18, help="sampled nodes checked against FP64")
     ap.add_argument("--check-batch", type=int, default=1024, help="sampled nodes per batch mesh")
     ap.add_argument("--check-serve", type=int, default=512, help="sampled points per mesh (served)")
     ap.add_argument("--check-beams", type=int, default=1 << 16, help="sampled queries per beam query set")
@@ -712,7 +723,7 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args, m, n, lo, hi, samples, errs)
         line["error_vs_fp64"] = errs
-        for key, leg in (("fixup_query_only", "c2_fixup"), ("plain", "c2_plain")):
+        for key, leg in (("fixup_query_only", "torus200k_fixup"), ("plain", "torus200k_plain")):
             line[key]["error_vs_fp64"] = errs.get(leg)
         if line.get("batch64"):
             line["batch64"]["error_vs_fp64"] = errs.get("batch64")

@@ -477,6 +477,168 @@ void orc_lbvh_counts(ptrdiff_t npoints, const float *x, const float *y, const fl
 
 /* Leaf order of the canonical LBVH: the triangle index of each leaf, i.e. the triangles stably
  * sorted by the Morton code of their FP32 box centre (the order a correct GPU sort must produce). */
+/* Replay of the GPU's wave-coherent walk (plain FP32) over the 4 x 4 x 4 bricks of a grid: per
+ * brick, 64 lanes share one walk of the same LBVH, whose references to subtrees of at most 3 leaves
+ * name runs of leaves that every lane evaluates.  At an internal node every lane tests both child
+ * boxes; the walk takes the child that the majority of the lanes wanting a child find nearer, and
+ * stacks the other one if some lane wants it; a popped entry is re-tested by every lane.  Lanes
+ * outside the grid take part with a bound of -inf.  Sums over the sampled bricks (stride `every` in
+ * brick order): node visits, pops, leaf references and triangle evaluations (each by 64 lanes).
+ * This is the packet walk's executed work, against which orc_lbvh_counts' canonical per-query work
+ * gives the fraction that is useful. */
+static int orc_subtree_leaves(const orc_lbvh *b, int *nl, int node) {
+    int c[2] = {b->left[node], b->right[node]}, n = 0;
+    for (int h = 0; h < 2; h++) n += c[h] < 0 ? 1 : orc_subtree_leaves(b, nl, c[h]);
+    return nl[node] = n;
+}
+
+void orc_packet_counts(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
+                       const float *sz, int nx, int ny, int nz, const float *lo, const float *hi, int every,
+                       int64_t *out) {
+    float *fmin_ = (float *)calloc(3 * nsel + 1, sizeof(float)), *fmax_ = (float *)calloc(3 * nsel + 1, sizeof(float));
+    for (ptrdiff_t t = 0; t < nsel; t++) {
+        const int iv[3] = {s0[t], s1[t], s2[t]};
+        const float *co[3] = {sx, sy, sz};
+        for (int d = 0; d < 3; d++) {
+            const float a = co[d][iv[0]], b_ = co[d][iv[1]], c = co[d][iv[2]];
+            fmin_[3 * t + d] = ORC_MIN3(a, b_, c), fmax_[3 * t + d] = ORC_MAX3(a, b_, c);
+        }
+    }
+    orc_lbvh b;
+    orc_lbvh_build(&b, nsel, fmin_, fmax_);
+    float *lbox = (float *)malloc(sizeof(float) * 6 * nsel);
+    for (ptrdiff_t k = 0; k < nsel; k++) {
+        const int t = b.order[k];
+        for (int d = 0; d < 3; d++) lbox[6 * k + d] = fmin_[3 * t + d], lbox[6 * k + 3 + d] = fmax_[3 * t + d];
+    }
+    free(fmin_), free(fmax_);
+    const ptrdiff_t ni = nsel > 1 ? nsel - 1 : 1;
+    float *nbox = (float *)malloc(sizeof(float) * 6 * ni);
+    orc_refit_f(&b, lbox, nbox);
+    int *nl = (int *)calloc((size_t)ni, sizeof(int));
+    int *first = (int *)calloc((size_t)ni, sizeof(int)); /* first leaf of each subtree (leaves in order) */
+    if (nsel > 1) {
+        orc_subtree_leaves(&b, nl, 0);
+        /* first leaf position: the leftmost leaf below */
+        for (ptrdiff_t i = 0; i < ni; i++) {
+            int c = (int)i;
+            while (c >= 0) c = b.left[c];
+            first[i] = ~c;
+        }
+    }
+    float h[3];
+    const int n3[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; d++) h[d] = n3[d] > 1 ? (hi[d] - lo[d]) / (float)(n3[d] - 1) : 0.0f;
+    const int nbx = (nx + 3) / 4, nby = (ny + 3) / 4, nbz = (nz + 3) / 4;
+    const ptrdiff_t nbricks = (ptrdiff_t)nbx * nby * nbz;
+    int64_t visits = 0, pops = 0, refs = 0, tris = 0, bricks = 0;
+#pragma omp parallel reduction(+ : visits, pops, refs, tris, bricks)
+    {
+        int st[256];
+#pragma omp for schedule(dynamic, 16)
+        for (ptrdiff_t bi = 0; bi < nbricks; bi += every) {
+            const int bx = (int)(bi % nbx), by = (int)((bi / nbx) % nby), bz = (int)(bi / ((ptrdiff_t)nbx * nby));
+            float px[64], py[64], pz[64], best[64];
+            for (int l = 0; l < 64; l++) {
+                const int i = bx * 4 + (l & 3), j = by * 4 + ((l >> 2) & 3), k = bz * 4 + (l >> 4);
+                const int ok = i < nx && j < ny && k < nz;
+                px[l] = lo[0] + (float)i * h[0], py[l] = lo[1] + (float)j * h[1], pz[l] = lo[2] + (float)k * h[2];
+                best[l] = ok ? INFINITY : -INFINITY;
+            }
+            bricks++;
+            int sp = 0;
+            /* references: >= 0 internal node to visit; < 0 a run ~(first leaf) with count cnt */
+            int node = nsel > 1 ? 0 : ~0, cnt = 1;
+#define ORC_REF(c, ref_, cnt_)                                \
+    do {                                                      \
+        if ((c) < 0) (ref_) = (c), (cnt_) = 1;                \
+        else if (nl[(c)] <= 3) (ref_) = ~first[(c)], (cnt_) = nl[(c)]; \
+        else (ref_) = (c), (cnt_) = 0;                        \
+    } while (0)
+            if (node == 0 && nl[0] <= 3) node = ~first[0], cnt = nl[0];
+            int stcnt[256];
+            for (;;) {
+                while (node >= 0) {
+                    visits++;
+                    const int c[2] = {b.left[node], b.right[node]};
+                    const float *cb[2];
+                    for (int hh = 0; hh < 2; hh++) cb[hh] = c[hh] < 0 ? lbox + 6 * (~c[hh]) : nbox + 6 * c[hh];
+                    int nlw = 0, nrw = 0, nlt = 0, nany = 0;
+                    for (int l = 0; l < 64; l++) {
+                        const float d0 = pt_box_d2_f(px[l], py[l], pz[l], cb[0][0], cb[0][3], cb[0][1], cb[0][4],
+                                                     cb[0][2], cb[0][5]);
+                        const float d1 = pt_box_d2_f(px[l], py[l], pz[l], cb[1][0], cb[1][3], cb[1][1], cb[1][4],
+                                                     cb[1][2], cb[1][5]);
+                        const int wl = d0 < best[l], wr = d1 < best[l];
+                        nlw += wl, nrw += wr, nlt += wl && (!wr || d0 <= d1), nany += wl || wr;
+                    }
+                    if (nany == 0) {
+                        node = INT32_MIN;
+                        break;
+                    }
+                    const int lf = 2 * nlt >= nany;
+                    const int take = lf ? c[0] : c[1], other = lf ? c[1] : c[0];
+                    if (lf ? nrw > 0 : nlw > 0) {
+                        int r, n;
+                        ORC_REF(other, r, n);
+                        st[sp] = r, stcnt[sp] = n, sp++;
+                    }
+                    ORC_REF(take, node, cnt);
+                }
+                if (node != INT32_MIN) { /* a run of cnt leaves from ~node */
+                    refs++;
+                    for (int j = 0; j < cnt; j++) {
+                        const int t = b.order[~node + j];
+                        const int i0 = s0[t], i1 = s1[t], i2 = s2[t];
+                        tris++;
+                        for (int l = 0; l < 64; l++) {
+                            const float d2 = pt_tri_d2_f(px[l], py[l], pz[l], sx[i0], sy[i0], sz[i0], sx[i1], sy[i1],
+                                                         sz[i1], sx[i2], sy[i2], sz[i2]);
+                            if (d2 < best[l]) best[l] = d2;
+                        }
+                    }
+                }
+                node = INT32_MIN;
+                while (sp > 0) {
+                    --sp;
+                    pops++;
+                    const int r = st[sp], n = stcnt[sp];
+                    /* the box of the pending reference: a node's, a leaf's, or a run's subtree's */
+                    const float *bx;
+                    if (r >= 0)
+                        bx = nbox + 6 * r;
+                    else if (n == 1)
+                        bx = lbox + 6 * (~r);
+                    else
+                        bx = NULL;
+                    float rb[6];
+                    if (!bx) { /* run: union of its leaves' boxes */
+                        for (int d = 0; d < 3; d++) rb[d] = INFINITY, rb[3 + d] = -INFINITY;
+                        for (int j = 0; j < n; j++)
+                            for (int d = 0; d < 3; d++) {
+                                rb[d] = ORC_MIN(rb[d], lbox[6 * (~r + j) + d]);
+                                rb[3 + d] = ORC_MAX(rb[3 + d], lbox[6 * (~r + j) + 3 + d]);
+                            }
+                        bx = rb;
+                    }
+                    int any = 0;
+                    for (int l = 0; l < 64 && !any; l++)
+                        any = pt_box_d2_f(px[l], py[l], pz[l], bx[0], bx[3], bx[1], bx[4], bx[2], bx[5]) < best[l];
+                    if (any) {
+                        node = r, cnt = n;
+                        break;
+                    }
+                }
+                if (node == INT32_MIN) break;
+            }
+#undef ORC_REF
+        }
+    }
+    out[0] = bricks, out[1] = visits, out[2] = pops, out[3] = refs, out[4] = tris;
+    free(lbox), free(nbox), free(nl), free(first);
+    orc_lbvh_free(&b);
+}
+
 void orc_lbvh_order(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
                     const float *sz, int *out_order) {
     float *fmin_ = (float *)calloc(3 * nsel + 1, sizeof(float)), *fmax_ = (float *)calloc(3 * nsel + 1, sizeof(float));

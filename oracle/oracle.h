@@ -105,6 +105,13 @@ void orc_lbvh_counts(ptrdiff_t npoints, const float *x, const float *y, const fl
                      int64_t *out_nbox, int64_t *out_ntri, int *out_maxdepth, float *out_d2);
 
 /* Leaf order of the canonical LBVH (triangle index per leaf: stable sort by Morton code). */
+/* Replay of the GPU's wave-coherent packet walk over the 4x4x4 bricks of an nx x ny x nz grid on
+ * [lo, hi] (every `every`-th brick): out[0..4] = bricks, node visits, pops, leaf references, triangle
+ * evaluations (each performed by all 64 lanes).  Parity unpinned: a model of this build's kernel for
+ * the roofline's useful-work fraction, not a reference algorithm. */
+void orc_packet_counts(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
+                       const float *sz, int nx, int ny, int nz, const float *lo, const float *hi, int every,
+                       int64_t *out);
 void orc_lbvh_order(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2, const float *sx, const float *sy,
                     const float *sz, int *out_order);
 

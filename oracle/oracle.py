@@ -46,6 +46,8 @@ def _load():
             "orc_sign": ([_PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_gwn": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_lbvh_counts": ([_PD, _P, _P, _P, _PD, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+            "orc_packet_counts": ([_PD, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, _P],
+                                  None),
             "orc_lbvh_order": ([_PD, _P, _P, _P, _P, _P, _P, _P], None),
             "orc_num_threads": ([], C.c_int),
         }
@@ -277,6 +279,20 @@ def lbvh_counts(x, y, z, mesh):
     L.orc_lbvh_counts(n, _p(x), _p(y), _p(z), s0.size, _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), _p(nb),
                       _p(nt), C.byref(md), _p(d2))
     return nb, nt, int(md.value), d2
+
+
+def packet_counts(mesh, n, lo, hi, every=1):
+    """Replay of the GPU's packet walk over the grid's 4x4x4 bricks (every `every`-th):
+    dict(bricks, visits, pops, leaf_refs, tri_evals)."""
+    L = _load()
+    sx, sy, sz = [np.ascontiguousarray(v, dtype=np.float32) for v in (mesh.sx, mesh.sy, mesh.sz)]
+    s0, s1, s2 = _i32(mesh.s0, mesh.s1, mesh.s2)
+    lo_ = np.ascontiguousarray(lo, dtype=np.float32)
+    hi_ = np.ascontiguousarray(hi, dtype=np.float32)
+    out = np.zeros(5, np.int64)
+    L.orc_packet_counts(s0.size, _p(s0), _p(s1), _p(s2), _p(sx), _p(sy), _p(sz), int(n[0]), int(n[1]), int(n[2]),
+                        _p(lo_), _p(hi_), int(every), _p(out))
+    return dict(zip(("bricks", "visits", "pops", "leaf_refs", "tri_evals"), (int(v) for v in out)))
 
 
 def lbvh_order(mesh):

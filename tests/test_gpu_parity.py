@@ -61,10 +61,10 @@ def test_reference_brute_force_case(gpu_lib, oracle):
 
 
 # ----------------------------------------------------------------------------------------------
-# config 1: UV sphere 48x24, 64^3 grid on [-1,1]^3
+# UV sphere 48x24, 64^3 grid on [-1,1]^3
 # ----------------------------------------------------------------------------------------------
 @pytest.fixture(scope="module")
-def c1(oracle):
+def sphere48(oracle):
     m = mg.uv_sphere()
     x, y, z = mg.grid_points(64, 64, 64, (-1, -1, -1), (1, 1, 1))
     s, tri, sg, rg, dot = oracle.sdf(x, y, z, m)
@@ -72,8 +72,8 @@ def c1(oracle):
     return m, (x, y, z), s, tri, d2
 
 
-def test_c1_fp64_edf_bit_exact(c1):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_fp64_edf_bit_exact(sphere48):
+    m, (x, y, z), s, tri, d2 = sphere48
     md = m.as_dtype(np.float64)
     out = np.full(x.size, 1e9)
     S.edf(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), md.s0, md.s1, md.s2, md.sx, md.sy, md.sz,
@@ -81,8 +81,8 @@ def test_c1_fp64_edf_bit_exact(c1):
     assert np.array_equal(out, np.sqrt(d2))
 
 
-def test_c1_fp64_closest_point_index(c1):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_fp64_closest_point_index(sphere48):
+    m, (x, y, z), s, tri, d2 = sphere48
     md = m.as_dtype(np.float64)
     n = x.size
     outd = np.full(n, np.inf)
@@ -97,33 +97,33 @@ def test_c1_fp64_closest_point_index(c1):
     assert np.abs(dd - outd).max() < 1e-12
 
 
-def test_c1_fp32_plain_edf(c1):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_fp32_plain_edf(sphere48):
+    m, (x, y, z), s, tri, d2 = sphere48
     out = np.full(x.size, np.finfo(np.float32).max, np.float32)
     S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
-    bars.plain_report(out, np.sqrt(d2), _diag(m), "c1 edf")
+    bars.plain_report(out, np.sqrt(d2), _diag(m), "sphere48 edf")
 
 
 @pytest.mark.parametrize("flags", [S.FLAG_FIXUP, S.FLAG_FIXUP | S.FLAG_SIGN_FP64])
-def test_c1_sdf_fixup_grid(c1, flags):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_sdf_fixup_grid(sphere48, flags):
+    m, (x, y, z), s, tri, d2 = sphere48
     out, otri, rm, nq = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), flags)
     assert np.array_equal(otri, tri), f"{(otri != tri).sum()} index mismatches"
-    bars.assert_fixup_distances(out, np.sqrt(d2), "c1 grid")
+    bars.assert_fixup_distances(out, np.sqrt(d2), "sphere48 grid")
     assert bars.sign_mismatches_outside_band(out, s, 1e-6 * _diag(m)) == 0
 
 
-def test_c1_sdf_plain_grid_sign_and_distance(c1):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_sdf_plain_grid_sign_and_distance(sphere48):
+    m, (x, y, z), s, tri, d2 = sphere48
     out, otri, rm, nq = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), 0)
     eps = 1e-6 * _diag(m)
-    bars.plain_report(out, np.sqrt(d2), _diag(m), "c1 sdf grid")
+    bars.plain_report(out, np.sqrt(d2), _diag(m), "sphere48 sdf grid")
     assert np.sum((np.sign(out) != np.sign(s)) & (np.sqrt(d2) > eps)) == 0
     assert nq == 0
 
 
-def test_c1_points_equal_grid(c1):
-    m, (x, y, z), s, tri, d2 = c1
+def test_sphere48_points_equal_grid(sphere48):
+    m, (x, y, z), s, tri, d2 = sphere48
     out_g, tri_g, rm, _ = _sdf_grid(m, (64, 64, 64), (-1, -1, -1), (1, 1, 1), S.FLAG_FIXUP)
     out_p = np.empty(x.size, np.float32)
     tri_p = np.empty(x.size, np.int32)
@@ -222,7 +222,7 @@ def test_deep_lbvh_two_vgpr_stack(oracle):
 
 
 # ----------------------------------------------------------------------------------------------
-# noisy torus (config 2's mesh family) at test size, and the staircase sign fixture
+# noisy torus (the 200 k benchmark torus at test size), and the staircase sign fixture
 # ----------------------------------------------------------------------------------------------
 def test_noisy_torus_fixup_parity(oracle):
     m = mg.torus(0.6, 0.2, 125, 50, sigma=0.005, seed=2)
