@@ -116,6 +116,8 @@ def run_gpu(args, m, n, lo, hi, samples):
     # one resident handle; every step rebuilds its LBVH and pseudonormals in place from the device
     # mesh arrays (no allocation) and runs the full query
     handle = create()
+    brick = [int(v) for v in args.brick.split(",")]
+    S._check(L.ssdf_mesh_set_schedule(handle, *brick, -1), "ssdf_mesh_set_schedule")
 
     def step(flags, qtimes):
         S._check(L.ssdf_mesh_rebuild(handle, 1, sp), "ssdf_mesh_rebuild")
@@ -227,23 +229,41 @@ def run_batch(args, samples):
     # the per-mesh calls and the batch agree bit for bit
     res["matches_per_mesh_calls"] = bool(all(np.array_equal(host[off[i]:off[i] + outs[i].size], outs[i])
                                              for i in range(len(meshes))))
-    bpc, ch, tune_ms = b.autotune(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
-    res["schedule"] = {"blocks_per_cu": bpc, "units_per_grab": ch, "query_ms": round(tune_ms, 3)}
     ev = [S.Event(), S.Event()]
-    for rebuild, key in ((True, "resident_step"), (False, "resident_query")):
-        b.capture(rebuild, S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+
+    def replay_ms(reps):
         for _ in range(max(1, args.warmup)):
             b.replay(st)
         st.synchronize()
         ev[0].record(st)
-        for _ in range(args.steps):
+        for _ in range(reps):
             b.replay(st)
         ev[1].record(st)
         st.synchronize()
-        ms = ev[0].elapsed_ms(ev[1]) / args.steps
+        return ev[0].elapsed_ms(ev[1]) / reps
+
+    # the default schedule, then the autotuned one (per mesh: brick shape and leaf budget; then the
+    # launch geometry)
+    b.capture(False, S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+    default_ms = replay_ms(args.steps)
+    log(f"[bench] batch64 resident_query (default schedule): {default_ms:.3f} ms")
+    bpc, ch, tune_ms = b.autotune(S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+    per_mesh = [b.mesh_schedule(i) for i in range(len(meshes))]
+    bricks, budgets = {}, {}
+    for (bk, hv) in per_mesh:
+        key = "x".join(map(str, bk))
+        bricks[key] = bricks.get(key, 0) + 1
+        budgets[str(hv)] = budgets.get(str(hv), 0) + 1
+    res["schedule"] = {"blocks_per_cu": bpc, "units_per_grab": ch, "query_ms": round(tune_ms, 3),
+                       "meshes_per_brick": bricks, "meshes_per_leaf_budget": budgets,
+                       "resident_query_default_schedule_ms": round(default_ms, 3)}
+    for rebuild, key in ((True, "resident_step"), (False, "resident_query")):
+        b.capture(rebuild, S.MODE_SDF, dims, boxes, off, out, tri, flags=flags, stream=st)
+        ms = replay_ms(args.steps)
         res[key + "_ms"] = ms
         res[key + "_mqps"] = nq / (ms * 1e-3) / 1e6
         log(f"[bench] batch64 {key}: {ms:.3f} ms")
+    res["schedule"]["autotune_gain"] = round(default_ms / res["resident_query_ms"], 4)
     q, e = b.last_stats()
     res["fixup_queued"], res["kernel_error"] = q, e
     # the batch query against the FP32 vector peak: canonical per-query work of every mesh's grid
@@ -653,6 +673,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--brick", default="4,4,4", help="grid nodes per wave of the torus legs (x,y,z; product 64)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 23)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg (and the FP64 checks)")
     ap.add_argument("--only", choices=["fixup", "plain", "batch"], default=None,

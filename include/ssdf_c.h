@@ -230,6 +230,12 @@ int ssdf_mesh_inside_points(ssdf_mesh_t mesh, ptrdiff_t npoints, const void *x, 
 int ssdf_mesh_inside_grid(ssdf_mesh_t mesh, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz, double xmin, double ymin,
                           double zmin, double xmax, double ymax, double zmax, uint8_t *out_inside, void *stream);
 
+/* Schedule of this handle's grid queries (no counterpart in the reference; a tuning knob): the
+ * brick of grid nodes one wave answers (brick_x x brick_y x brick_z = 64, powers of two; default 4 x
+ * 4 x 4) and the leaf budget of its packet waves (-1: the process-wide one, ssdf_set_heavy_leaves).
+ * FP32 + fixup results do not depend on either. */
+int ssdf_mesh_set_schedule(ssdf_mesh_t mesh, int brick_x, int brick_y, int brick_z, int heavy_leaves);
+
 /* Statistics of the last query on this handle (after its stream has completed): number of
  * queries re-traversed in FP64 because their candidate list overflowed, and the kernel error flag:
  * bit 1 a traversal stack overflowed, bit 2 the iteration bound was hit (malformed tree), bit 4
@@ -265,8 +271,20 @@ int ssdf_batch_query_points(ssdf_batch_t batch, int mode, const ptrdiff_t *point
                             float *out_cz, unsigned flags, void *stream);
 /* Persistent-launch schedule: blocks per CU (grid size) and work units per queue grab; 0 = default. */
 int ssdf_batch_set_schedule(ssdf_batch_t batch, int blocks_per_cu, int units_per_grab);
-/* Times the grid query over a small schedule space (blocks per CU x units per grab), keeps the
- * fastest and reports it.  Runs the query 40 times. */
+/* Per-mesh schedule of grid queries: the brick of grid nodes one wave answers (brick_x x brick_y x
+ * brick_z = 64, powers of two; default 4 x 4 x 4) and the leaf budget of the mesh's packet waves
+ * before they hand their queries to the per-query kernel (-1: the process-wide budget,
+ * ssdf_set_heavy_leaves; 0: never).  FP32 + fixup results do not depend on either. */
+int ssdf_batch_set_mesh_schedule(ssdf_batch_t batch, int mesh, int brick_x, int brick_y, int brick_z,
+                                 int heavy_leaves);
+int ssdf_batch_get_mesh_schedule(ssdf_batch_t batch, int mesh, int *brick_x, int *brick_y, int *brick_z,
+                                 int *heavy_leaves);
+/* Autotuner of grid queries.  Per mesh, the brick (4x4x4, 8x4x2, 4x8x2, 2x4x8, 8x8x1), chosen from
+ * one query per candidate in which every mesh's waves clock their own cycles (packet and per-query
+ * kernels), so each mesh keeps its cheapest.  Then the leaf budget (4096, 8192, 16384) and the launch
+ * (blocks per CU x units per grab), timed with events under the per-mesh bricks.  Keeps the result
+ * (ssdf_batch_get_mesh_schedule reads it) and reports the launch schedule and its time.  Runs the
+ * query 52 times. */
 int ssdf_batch_autotune(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, const double *boxes,
                         const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
                         int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms);

@@ -93,6 +93,7 @@ _SIGS = {
     "ssdf_mesh_query_grid": [_P, C.c_int, _PD, _PD, _PD, C.c_double, C.c_double, C.c_double, C.c_double,
                              C.c_double, C.c_double, _P, _P, C.c_uint, _P],
     "ssdf_mesh_last_stats": [_P, C.POINTER(C.c_uint), C.POINTER(C.c_int)],
+    "ssdf_mesh_set_schedule": [_P, C.c_int, C.c_int, C.c_int, C.c_int],
     "ssdf_mesh_brute_points": [_P, _PD, _P, _P, _P, _P, _P, _P],
     "ssdf_mesh_leaf_order": [_P, _P, _P],
     "ssdf_batch_create": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(_P)],
@@ -101,6 +102,9 @@ _SIGS = {
     "ssdf_batch_query_grids": [_P, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P],
     "ssdf_batch_query_points": [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint, _P],
     "ssdf_batch_set_schedule": [_P, C.c_int, C.c_int],
+    "ssdf_batch_set_mesh_schedule": [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "ssdf_batch_get_mesh_schedule": [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)],
     "ssdf_batch_autotune": [_P, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                             C.POINTER(C.c_float)],
     "ssdf_batch_capture": [_P, C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_uint, _P],
@@ -531,6 +535,11 @@ class ResidentMesh:
         _check(lib().ssdf_mesh_last_stats(C.c_void_p(self.h), C.byref(q), C.byref(e)), "ssdf_mesh_last_stats")
         return int(q.value), int(e.value)
 
+    def set_schedule(self, brick=(4, 4, 4), heavy_leaves: int = -1):
+        """Brick of grid nodes per wave (product 64, powers of two) and leaf budget (-1: process-wide)."""
+        _check(lib().ssdf_mesh_set_schedule(C.c_void_p(self.h), int(brick[0]), int(brick[1]), int(brick[2]),
+                                            int(heavy_leaves)), "ssdf_mesh_set_schedule")
+
     def close(self):
         if getattr(self, "h", None):
             lib().ssdf_mesh_destroy(C.c_void_p(self.h))
@@ -598,6 +607,19 @@ class Batch:
     def set_schedule(self, blocks_per_cu: int, units_per_grab: int):
         _check(lib().ssdf_batch_set_schedule(C.c_void_p(self.h), blocks_per_cu, units_per_grab),
                "ssdf_batch_set_schedule")
+
+    def set_mesh_schedule(self, mesh: int, brick=(4, 4, 4), heavy_leaves: int = -1):
+        """Brick of grid nodes per wave (product 64, powers of two) and leaf budget (-1: process-wide)
+        of one mesh's grid queries."""
+        _check(lib().ssdf_batch_set_mesh_schedule(C.c_void_p(self.h), mesh, int(brick[0]), int(brick[1]),
+                                                  int(brick[2]), int(heavy_leaves)), "ssdf_batch_set_mesh_schedule")
+
+    def mesh_schedule(self, mesh: int):
+        """((brick_x, brick_y, brick_z), heavy_leaves) of one mesh."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().ssdf_batch_get_mesh_schedule(C.c_void_p(self.h), mesh, *[C.byref(x) for x in v]),
+               "ssdf_batch_get_mesh_schedule")
+        return (v[0].value, v[1].value, v[2].value), v[3].value
 
     def autotune(self, mode, dims, boxes, out_offsets, out_d, out_tri=None, flags=0, stream: Stream | None = None):
         d = np.ascontiguousarray(dims, np.int64).reshape(-1)

@@ -52,6 +52,9 @@ struct ssdf_batch {
     int maxdepth = 0;
     int num_cus = 256;
     int blocks_per_cu = 0, chunk = 0;  // schedule (0: defaults)
+    std::vector<int> mesh_brick;       // per mesh: brick shape of its grid waves (brick_code; 0: 4x4x4)
+    std::vector<int> mesh_heavy;       // per mesh: leaf budget of its packet waves (-1: process-wide)
+    unsigned long long *cost = nullptr;  // per-mesh cycle counters of the autotuner
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
 };
@@ -69,7 +72,7 @@ void destroy_batch(ssdf_batch *b) {
     for (void *p : {(void *)b->s0, (void *)b->s1, (void *)b->s2, (void *)b->sx, (void *)b->sy, (void *)b->sz,
                     (void *)b->nodes, (void *)b->tris, (void *)b->metas, (void *)b->face_n, (void *)b->edge_n,
                     (void *)b->vertex_n, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
-                    (void *)b->fix_list})
+                    (void *)b->fix_list, (void *)b->cost})
         dev_free(p, st);
     for (int l = 1; l < ssdf_batch::kLanes; l++) dev_free(b->lane_ws[l], st);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
@@ -202,6 +205,8 @@ int plan_queries(ssdf_batch *b, int qsrc, const ptrdiff_t *dims, const double *b
         BatchMesh d{};
         d.node_off = b->node_off[i], d.tri_off = b->tri_off[i], d.vert_off = b->vert_off[i];
         d.mesh = i;
+        d.brick = b->mesh_brick[i];
+        d.heavy = b->mesh_heavy[i];
         const float L = meta_scale(m.hmeta);
         d.band_abs = kBandRel * L;
         d.sign_eps = (float)(kSignEpsRel * meta_diag(m.hmeta));
@@ -221,9 +226,9 @@ int plan_queries(ssdf_batch *b, int qsrc, const ptrdiff_t *dims, const double *b
                 d.gmin[k] = l;
                 d.gh[k] = n[k] > 1 ? (double)((h - l) / (float)(n[k] - 1)) : 0.0;
             }
-            d.nbx = (d.nx + 3) / 4, d.nby = (d.ny + 3) / 4, d.nbz = (d.nz + 3) / 4;
-            const unsigned long long ns = (unsigned long long)((d.nbx + 3) / 4) * ((d.nby + 3) / 4) * ((d.nbz + 3) / 4);
-            if (m.nsel > 0 && d.nx > 0 && d.ny > 0 && d.nz > 0) u = ns * 64;
+            unsigned long long w = 0;
+            grid_bricks(d.brick, d.nx, d.ny, d.nz, d.nbx, d.nby, d.nbz, w);
+            if (m.nsel > 0 && d.nx > 0 && d.ny > 0 && d.nz > 0) u = w;
             p.total_queries += (size_t)n[0] * n[1] * n[2];
         } else {
             d.out_off = offsets[i];
@@ -291,7 +296,7 @@ int stage(ssdf_batch *b, const Plan &p, int qsrc, int mode, unsigned flags, cons
     a.nmeshes = (int)p.desc.size();
     a.total_units = p.total_units;
     a.queue = b->queue;
-    a.chunk = (unsigned)(b->chunk > 0 ? b->chunk : 2);
+    a.chunk = (unsigned)(b->chunk > 0 ? b->chunk : 1);
     (void)qsrc;
     (void)mode;
     return SSDF_OK;
@@ -303,15 +308,18 @@ int stack_of(const ssdf_batch *b) {
     return 0;
 }
 
+// Default: 8 blocks per CU (8 waves per SIMD, the packet kernel's occupancy), what the autotuner
+// picks on the 64-mesh batch in both modes.
 unsigned grid_blocks(const ssdf_batch *b, unsigned total_units, bool fix) {
-    const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu : (fix ? 5 : 7);
+    (void)fix;
+    const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu : 8;
     const unsigned want = (unsigned)(b->num_cus * bpc);
     return std::max(1u, std::min(want, (total_units + 3) / 4));
 }
 
 int run_query(ssdf_batch *b, int qsrc, int mode, const ptrdiff_t *dims, const double *boxes,
               const ptrdiff_t *offsets, const float *x, const float *y, const float *z, float *out_d, int *out_tri,
-              float *cx, float *cy, float *cz, unsigned flags, hipStream_t st) {
+              float *cx, float *cy, float *cz, unsigned flags, hipStream_t st, unsigned long long *cost = nullptr) {
     if (mode == SSDF_MODE_SDF && !b->with_normals) {
         set_error("batch: signed queries need the meshes' pseudonormals");
         return SSDF_ERR_INVALID;
@@ -331,7 +339,29 @@ int run_query(ssdf_batch *b, int qsrc, int mode, const ptrdiff_t *dims, const do
         return SSDF_ERR_DEPTH;
     }
     const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
+    a.cost = cost;
     return launch_batch(a, b->d_desc, qsrc, mode, fix, stack, grid_blocks(b, p.total_units, fix), st);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// One grid query with every mesh's waves timed (shader-clock cycles summed per mesh, packet and
+// per-query kernels alike) into cost[nm] on the host.
+int timed_costs(ssdf_batch *b, int mode, const ptrdiff_t *dims, const double *boxes, const ptrdiff_t *out_offsets,
+                float *out_d, int *out_tri, unsigned flags, hipStream_t st, std::vector<unsigned long long> &cost) {
+    if (!b->cost) SSDF_HIP_TRY(dev_alloc((void **)&b->cost, sizeof(unsigned long long) * b->nm, b->stream));
+    if (st != b->stream) SSDF_HIP_TRY(hipStreamSynchronize(b->stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3((b->nm + 255) / 256), dim3(256), 0, st, b->cost, b->nm);
+    if (int rc = run_query(b, QSRC_GRID, mode, dims, boxes, out_offsets, nullptr, nullptr, nullptr, out_d, out_tri,
+                           nullptr, nullptr, nullptr, flags, st, b->cost))
+        return rc;
+    cost.assign(b->nm, 0ull);
+    SSDF_HIP_TRY(hipMemcpyAsync(cost.data(), b->cost, sizeof(unsigned long long) * b->nm, hipMemcpyDeviceToHost, st));
+    SSDF_HIP_TRY(hipStreamSynchronize(st));
+    return SSDF_OK;
 }
 
 }  // namespace
@@ -414,6 +444,8 @@ int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t
         SSDF_HIP_TRY(hipMemcpyAsync(b->sz, sz, sizeof(float) * V, kind, st));
     }
     b->meshes.resize(nmeshes);
+    b->mesh_brick.assign(nmeshes, 0);
+    b->mesh_heavy.assign(nmeshes, -1);
     for (int i = 0; i < nmeshes; i++) {
         DevMesh &m = b->meshes[i];
         m.dtype = 0;
@@ -492,6 +524,40 @@ int ssdf_batch_set_schedule(ssdf_batch_t b, int blocks_per_cu, int units_per_gra
     return SSDF_OK;
 }
 
+int ssdf_batch_set_mesh_schedule(ssdf_batch_t b, int mesh, int brick_x, int brick_y, int brick_z, int heavy_leaves) {
+    if (!b || mesh < 0 || mesh >= b->nm || heavy_leaves < -1) {
+        set_error("ssdf_batch_set_mesh_schedule: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    int lg[3] = {-1, -1, -1};
+    const int ext[3] = {brick_x, brick_y, brick_z};
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k <= 6; k++)
+            if (ext[d] == (1 << k)) lg[d] = k;
+    if (lg[0] < 0 || lg[1] < 0 || lg[2] < 0 || lg[0] + lg[1] + lg[2] != 6) {
+        set_error("ssdf_batch_set_mesh_schedule: the brick is 2^a x 2^b x 2^c = 64 nodes");
+        return SSDF_ERR_INVALID;
+    }
+    b->mesh_brick[mesh] = brick_code(lg[0], lg[1], lg[2]);
+    b->mesh_heavy[mesh] = heavy_leaves;
+    return SSDF_OK;
+}
+
+int ssdf_batch_get_mesh_schedule(ssdf_batch_t b, int mesh, int *brick_x, int *brick_y, int *brick_z,
+                                 int *heavy_leaves) {
+    if (!b || mesh < 0 || mesh >= b->nm) {
+        set_error("ssdf_batch_get_mesh_schedule: invalid arguments");
+        return SSDF_ERR_INVALID;
+    }
+    int bx, by, bz;
+    brick_log2(b->mesh_brick[mesh], bx, by, bz);
+    if (brick_x) *brick_x = 1 << bx;
+    if (brick_y) *brick_y = 1 << by;
+    if (brick_z) *brick_z = 1 << bz;
+    if (heavy_leaves) *heavy_leaves = b->mesh_heavy[mesh];
+    return SSDF_OK;
+}
+
 int ssdf_batch_autotune(ssdf_batch_t b, int mode, const ptrdiff_t *dims, const double *boxes,
                         const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
                         int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms) {
@@ -500,34 +566,72 @@ int ssdf_batch_autotune(ssdf_batch_t b, int mode, const ptrdiff_t *dims, const d
         return SSDF_ERR_INVALID;
     }
     const hipStream_t st = (hipStream_t)stream;
-    const int bpcs[] = {3, 4, 5, 6, 8};
-    const int chunks[] = {1, 2, 4, 8};
     hipEvent_t e0, e1;
     SSDF_HIP_TRY(hipEventCreate(&e0));
     SSDF_HIP_TRY(hipEventCreate(&e1));
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } events{e0, e1};
+    // one untimed run, then a timed one (events on the stream)
+    auto timed = [&](float &ms) -> int {
+        for (int rep = 0; rep < 2; rep++) {
+            if (rep == 1) (void)hipEventRecord(e0, st);
+            if (int rc = ssdf_batch_query_grids(b, mode, dims, boxes, out_offsets, out_d, out_tri, flags, stream))
+                return rc;
+            if (rep == 1) (void)hipEventRecord(e1, st);
+        }
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return SSDF_OK;
+    };
+    // 1. per mesh, the brick shape of its grid waves: one query per candidate with every mesh at
+    //    that candidate, each mesh's cost read from its waves' clocks (packet and per-query
+    //    kernels); every mesh keeps its cheapest.  FP32 + fixup results do not depend on it.
+    static const int shapes[] = {brick_code(2, 2, 2), brick_code(3, 2, 1), brick_code(2, 3, 1),
+                                 brick_code(1, 2, 3), brick_code(3, 3, 0)};
+    std::vector<unsigned long long> cost, best_cost(b->nm, ~0ull);
+    std::vector<int> best_shape(b->nm, 0);
+    std::fill(b->mesh_heavy.begin(), b->mesh_heavy.end(), -1);
+    int rc = run_query(b, QSRC_GRID, mode, dims, boxes, out_offsets, nullptr, nullptr, nullptr, out_d, out_tri, nullptr,
+                       nullptr, nullptr, flags, st);  // untimed first run (code objects, caches, workspace)
+    for (int sh : shapes) {
+        if (rc) break;
+        std::fill(b->mesh_brick.begin(), b->mesh_brick.end(), sh);
+        rc = timed_costs(b, mode, dims, boxes, out_offsets, out_d, out_tri, flags, st, cost);
+        for (int i = 0; i < b->nm && !rc; i++)
+            if (cost[i] < best_cost[i]) best_cost[i] = cost[i], best_shape[i] = sh;
+    }
+    if (rc) return rc;
+    b->mesh_brick = best_shape;
+    // 2. the leaf budget, by the launch's time: a mesh's summed wave cycles do not show the tail of
+    //    its longest waves, which the budget trades against the per-query kernel's work
+    static const int budgets[] = {4096, 8192, 16384};
+    float best_b = 1e30f;
+    int bh = -1;
+    for (int hb : budgets) {
+        std::fill(b->mesh_heavy.begin(), b->mesh_heavy.end(), hb);
+        float ms = 0.0f;
+        if ((rc = timed(ms))) return rc;
+        if (ms < best_b) best_b = ms, bh = hb;
+    }
+    std::fill(b->mesh_heavy.begin(), b->mesh_heavy.end(), bh);
+    // 3. the launch: blocks per CU x units per grab, timed by events with the per-mesh schedule
+    const int bpcs[] = {3, 4, 5, 6, 8};
+    const int chunks[] = {1, 2, 4, 8};
     float best = 1e30f;
     int bb = 0, bc = 0;
-    int rc = SSDF_OK;
     for (int bpc : bpcs) {
         for (int ch : chunks) {
             b->blocks_per_cu = bpc, b->chunk = ch;
-            // one untimed run, then the timed one
-            for (int rep = 0; rep < 2 && !rc; rep++) {
-                if (rep == 1) (void)hipEventRecord(e0, st);
-                rc = ssdf_batch_query_grids(b, mode, dims, boxes, out_offsets, out_d, out_tri, flags, stream);
-                if (rep == 1) (void)hipEventRecord(e1, st);
-            }
-            if (rc) break;
             float ms = 0.0f;
-            (void)hipEventSynchronize(e1);
-            (void)hipEventElapsedTime(&ms, e0, e1);
+            if ((rc = timed(ms))) return rc;
             if (ms < best) best = ms, bb = bpc, bc = ch;
         }
-        if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
     b->blocks_per_cu = bb, b->chunk = bc;
     if (best_blocks_per_cu) *best_blocks_per_cu = bb;
     if (best_units_per_grab) *best_units_per_grab = bc;

@@ -6,8 +6,10 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 #include <string>
 
 using namespace ssdf_amd;
@@ -135,6 +137,8 @@ struct ssdf_mesh {
     hipStream_t last_stream = nullptr;  // stream of the last query (its workspace use)
     bool queried = false;
     hipEvent_t order_ev = nullptr;  // orders one stream after another (stream_after)
+    int brick = 0;                  // brick shape of grid waves (brick_code; 0: 4 x 4 x 4)
+    int heavy = -1;                 // leaf budget of packet waves (-1: the process-wide one)
 };
 
 namespace {
@@ -351,6 +355,8 @@ int query_grid(ssdf_mesh *h, int mode, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_t nz,
     QueryArgs a{};
     fill_args(h->m, a, w, stream);
     setup_grid(h->m, a, nx, ny, nz, lo, hi);
+    a.brick = h->brick;
+    if (h->heavy >= 0) a.heavy_iters = h->heavy;
     a.out_d = out_d;
     a.out_tri = out_tri;
     const int dbl = h->m.dtype;
@@ -596,6 +602,48 @@ int host_inside_grid(ptrdiff_t nsel, const int *s0, const int *s1, const int *s2
     return d2h(out, dout, (ptrdiff_t)nq);
 }
 
+// SSDF_CELL_VALIDATE (serial/ssdf.hpp:979-1010), read on every call as the reference reads it: the
+// engine's result is checked against the exact FP64 argmin of the same inputs, and the maximum and
+// the norm of the relative differences are printed to stderr when the maximum exceeds 1e-5; the
+// engine's result is returned.
+int cell_validate() {
+    const char *e = std::getenv("SSDF_CELL_VALIDATE");
+    return e ? std::atoi(e) : 0;
+}
+
+template <typename T>
+int edf_validated(ptrdiff_t npoints, const T *x, const T *y, const T *z, ptrdiff_t nsel, const int *s0,
+                  const int *s1, const int *s2, ptrdiff_t nsp, const T *sx, const T *sy, const T *sz, T *out) {
+    if (npoints <= 0 || !out) return host_points_call<T>(SSDF_MODE_EDF, npoints, x, y, z, nsel, s0, s1, s2, nsp, sx, sy,
+                                                         sz, out, nullptr, nullptr, nullptr, nullptr, 0u);
+    std::vector<T> test(out, out + npoints);
+    if (int rc = host_points_call<T>(SSDF_MODE_EDF, npoints, x, y, z, nsel, s0, s1, s2, nsp, sx, sy, sz, test.data(),
+                                     nullptr, nullptr, nullptr, nullptr, 0u))
+        return rc;
+    // the exact FP64 search on the same values widened to FP64, seeded with the caller's out
+    std::vector<double> X(x, x + npoints), Y(y, y + npoints), Z(z, z + npoints);
+    std::vector<double> SX(sx, sx + (nsp > 0 ? nsp : 0)), SY(sy, sy + (nsp > 0 ? nsp : 0)),
+        SZ(sz, sz + (nsp > 0 ? nsp : 0));
+    std::vector<double> ref(out, out + npoints);
+    if (int rc = host_points_call<double>(SSDF_MODE_EDF, npoints, X.data(), Y.data(), Z.data(), nsel, s0, s1, s2, nsp,
+                                          SX.data(), SY.data(), SZ.data(), ref.data(), nullptr, nullptr, nullptr,
+                                          nullptr, 0u))
+        return rc;
+    T max_diff = 0, diff_norm = 0;
+    for (ptrdiff_t i = 0; i < npoints; ++i) {
+        const T r = (T)ref[i];
+        const T rel = std::abs(r - test[i]) / (std::abs(r) + (T)1e-16);
+        max_diff = std::max<T>(max_diff, rel);
+        diff_norm += rel * rel;
+    }
+    if (max_diff > (T)1e-5) {
+        std::fprintf(stderr, "EDF cell-list validation max rel diff: %g\n", (double)max_diff);
+        std::fprintf(stderr, "EDF cell-list validation rel diff norm: %g\n", (double)std::sqrt(diff_norm));
+    }
+    std::copy(test.begin(), test.end(), out);
+    return SSDF_OK;
+}
+
 }  // namespace
 
 namespace ssdf_amd {
@@ -714,6 +762,8 @@ int ssdf_mesh_inside_grid(ssdf_mesh_t mesh, ptrdiff_t nx, ptrdiff_t ny, ptrdiff_
 int ssdf_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z, ptrdiff_t nselements,
                const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const float *sx, const float *sy,
                const float *sz, float *out) {
+    if (cell_validate())
+        return edf_validated<float>(npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz, out);
     return host_points_call<float>(SSDF_MODE_EDF, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz,
                                    out, nullptr, nullptr, nullptr, nullptr, 0u);
 }
@@ -721,6 +771,8 @@ int ssdf_edf_f(ptrdiff_t npoints, const float *x, const float *y, const float *z
 int ssdf_edf_d(ptrdiff_t npoints, const double *x, const double *y, const double *z, ptrdiff_t nselements,
                const int *s0, const int *s1, const int *s2, ptrdiff_t nspoints, const double *sx, const double *sy,
                const double *sz, double *out) {
+    if (cell_validate())
+        return edf_validated<double>(npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz, out);
     return host_points_call<double>(SSDF_MODE_EDF, npoints, x, y, z, nselements, s0, s1, s2, nspoints, sx, sy, sz,
                                     out, nullptr, nullptr, nullptr, nullptr, 0u);
 }
@@ -859,6 +911,21 @@ int ssdf_mesh_query_grid(ssdf_mesh_t mesh, int mode, ptrdiff_t nx, ptrdiff_t ny,
     }
     const double lo[3] = {xmin, ymin, zmin}, hi[3] = {xmax, ymax, zmax};
     return query_grid(mesh, mode, nx, ny, nz, lo, hi, out_d, out_tri, flags, (hipStream_t)stream);
+}
+
+int ssdf_mesh_set_schedule(ssdf_mesh_t mesh, int brick_x, int brick_y, int brick_z, int heavy_leaves) {
+    int lg[3] = {-1, -1, -1};
+    const int ext[3] = {brick_x, brick_y, brick_z};
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k <= 6; k++)
+            if (ext[d] == (1 << k)) lg[d] = k;
+    if (!mesh || heavy_leaves < -1 || lg[0] < 0 || lg[1] < 0 || lg[2] < 0 || lg[0] + lg[1] + lg[2] != 6) {
+        set_error("ssdf_mesh_set_schedule: the brick is 2^a x 2^b x 2^c = 64 nodes, heavy_leaves >= -1");
+        return SSDF_ERR_INVALID;
+    }
+    mesh->brick = brick_code(lg[0], lg[1], lg[2]);
+    mesh->heavy = heavy_leaves;
+    return SSDF_OK;
 }
 
 int ssdf_mesh_last_stats(ssdf_mesh_t mesh, unsigned *fixup_queued, int *kernel_error) {

@@ -350,9 +350,13 @@ int launch_gwn(DevMesh &m, QueryArgs &a, int qsrc, int mode, int q_is_double, hi
     if (qsrc == QSRC_POINTS) {
         nwaves = (unsigned)((nq + 63) / 64);
     } else {
-        a.nbx = (a.nx + 3) / 4, a.nby = (a.ny + 3) / 4, a.nbz = (a.nz + 3) / 4;
-        const unsigned nsx = (a.nbx + 3) / 4, nsy = (a.nby + 3) / 4, nsz = (a.nbz + 3) / 4;
-        nwaves = nsx * nsy * nsz * 64;
+        unsigned long long w = 0;
+        grid_bricks(a.brick, a.nx, a.ny, a.nz, a.nbx, a.nby, a.nbz, w);
+        if (w > 0xffffffffull - kGwnWaves) {
+            set_error("winding-number grid: too many nodes for one launch");
+            return SSDF_ERR_INVALID;
+        }
+        nwaves = (unsigned)w;
     }
     const unsigned nblocks = (nwaves + kGwnWaves - 1) / kGwnWaves;
     a.nblocks = nblocks;

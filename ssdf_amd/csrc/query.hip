@@ -932,6 +932,8 @@ __device__ __forceinline__ void batch_args(const BatchArgs &b, const BatchMesh &
     a.qz = b.qz ? b.qz + bm.out_off : nullptr;
     a.npoints = bm.npoints;
     a.nx = bm.nx, a.ny = bm.ny, a.nz = bm.nz, a.nbx = bm.nbx, a.nby = bm.nby, a.nbz = bm.nbz;
+    a.brick = bm.brick;
+    if (bm.heavy >= 0) a.heavy_iters = bm.heavy;
 #pragma unroll
     for (int d = 0; d < 3; d++) a.gmin[d] = bm.gmin[d], a.gh[d] = bm.gh[d];
     a.out_d = b.out_d + bm.out_off;
@@ -980,8 +982,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 const BatchMesh &bm = meshes[m0];
                 QueryArgs a;
                 batch_args(b, bm, a);
+                const unsigned long long t0 = b.cost ? clock64() : 0ull;
                 packet_wave<STACK, QSRC, OUT, FIX>(nodes + bm.node_off, tris + bm.tri_off, a, u - bm.unit_start,
                                                    cand);
+                if (b.cost && lane == 0) atomicAdd(b.cost + bm.mesh, (unsigned long long)(clock64() - t0));
             }
         }
     }
@@ -1000,7 +1004,9 @@ __global__ __launch_bounds__(256) void k_batch_coop(const BatchMesh *__restrict_
         while (j < b.nmeshes - 1 && meshes[j].mesh != mi) j++;
         QueryArgs a;
         batch_args(b, meshes[j], a);
+        const unsigned long long t0 = b.cost ? clock64() : 0ull;
         coop_item<QSRC, OUT, FIX>(a, e, lev, nlev);
+        if (b.cost && (threadIdx.x & 63) == 0) atomicAdd(b.cost + mi, (unsigned long long)(clock64() - t0));
     }
 }
 
@@ -1187,9 +1193,13 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
     if (qsrc == QSRC_POINTS) {
         nwaves = (unsigned)((nq + 63) / 64);
     } else {
-        a.nbx = (a.nx + 3) / 4, a.nby = (a.ny + 3) / 4, a.nbz = (a.nz + 3) / 4;
-        const unsigned nsx = (a.nbx + 3) / 4, nsy = (a.nby + 3) / 4, nsz = (a.nbz + 3) / 4;
-        nwaves = nsx * nsy * nsz * 64;
+        unsigned long long w = 0;
+        grid_bricks(a.brick, a.nx, a.ny, a.nz, a.nbx, a.nby, a.nbz, w);
+        if (w > 0xffffffffull - kWaves) {
+            set_error("grid query: too many nodes for one launch");
+            return SSDF_ERR_INVALID;
+        }
+        nwaves = (unsigned)w;
     }
     const unsigned nblocks = (nwaves + kWaves - 1) / kWaves;
     int rc;

@@ -39,7 +39,8 @@ struct QueryArgs {
     const unsigned *perm;  // optional: wave slot j answers point perm[j] (Morton order)
     // ... or grid nodes x_i = gmin + i * gh (x fastest)
     int nx, ny, nz;
-    int nbx, nby, nbz;  // 4x4x4 bricks per axis
+    int nbx, nby, nbz;  // bricks per axis
+    int brick;          // brick shape of a wave's 64 grid nodes (brick_code; 0: 4 x 4 x 4)
     double gmin[3], gh[3];
     // outputs (caller precision T)
     void *out_d;
@@ -71,6 +72,28 @@ struct QueryArgs {
 
 int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,
                  int q_is_double, hipStream_t stream);
+
+// Brick shapes: a wave answers 2^bx x 2^by x 2^bz grid nodes (bx + by + bz = 6), coded
+// bx | by << 4 | bz << 8; code 0 is the default 4 x 4 x 4.
+__host__ __device__ constexpr int brick_code(int bx, int by, int bz) { return bx | by << 4 | bz << 8; }
+__host__ __device__ __forceinline__ void brick_log2(int code, int &bx, int &by, int &bz) {
+    if (code == 0) code = brick_code(2, 2, 2);
+    bx = code & 15, by = (code >> 4) & 15, bz = (code >> 8) & 15;
+}
+inline bool brick_valid(int code) {
+    if (code == 0) return true;
+    int bx, by, bz;
+    brick_log2(code, bx, by, bz);
+    return bx + by + bz == 6 && code == brick_code(bx, by, bz);
+}
+// Bricks per axis and waves of a grid under a brick shape: bricks are grouped 4 x 4 x 4 into
+// super-bricks (64 waves, Morton order inside), so the waves cover whole super-bricks.
+inline void grid_bricks(int code, int nx, int ny, int nz, int &nbx, int &nby, int &nbz, unsigned long long &waves) {
+    int bx, by, bz;
+    brick_log2(code, bx, by, bz);
+    nbx = (nx + (1 << bx) - 1) >> bx, nby = (ny + (1 << by) - 1) >> by, nbz = (nz + (1 << bz) - 1) >> bz;
+    waves = (unsigned long long)((nbx + 3) / 4) * ((nby + 3) / 4) * ((nbz + 3) / 4) * 64;
+}
 
 // Precision constants of the queries (single meshes and batches alike).
 constexpr float kBandRel = 0x1p-18f;    // FP32 error bound of a distance, relative to the coordinate scale
@@ -119,6 +142,8 @@ struct BatchMesh {
     double delta64;
     unsigned unit_start;  // first work unit
     int mesh;             // index of the mesh in the batch (meta record, fixup tag)
+    int brick;            // grid mode: brick shape (brick_code) of this mesh's waves
+    int heavy;            // leaf budget of this mesh's packet waves (< 0: the process-wide one)
 };
 
 struct BatchArgs {
@@ -138,6 +163,7 @@ struct BatchArgs {
     unsigned *queue;  // 8 counters (one per XCD range), 64 words apart
     unsigned chunk;   // units per grab
     int nlev;         // LDS levels of the per-query kernel: the deepest tree of the batch
+    unsigned long long *cost;  // optional, per mesh: shader-clock cycles its waves spent (autotuning)
 };
 
 // Launches the reset, the persistent packet kernel and the queue kernel of a batch.
@@ -226,9 +252,9 @@ __device__ __forceinline__ void store_as(void *p, long long i, T v) {
     ((T *)p)[i] = v;
 }
 
-// Query of a lane.  Grid: waves map to 4x4x4 bricks, bricks to 4x4x4 super-bricks in Morton order
-// (spatially compact waves and blocks), lanes to (lane&3, lane>>2&3, lane>>4).  Points: 64
-// consecutive points per wave.
+// Query of a lane.  Grid: waves map to bricks of 2^bx x 2^by x 2^bz nodes (4 x 4 x 4 by default),
+// bricks to 4x4x4 super-bricks in Morton order (spatially compact waves and blocks), lanes x fastest
+// inside the brick.  Points: 64 consecutive points per wave.
 template <int QSRC, typename Q>
 __device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id, int lane, long long &qid, Q &x,
                                            Q &y, Q &z) {
@@ -244,9 +270,11 @@ __device__ __forceinline__ bool lane_query(const QueryArgs &a, unsigned wave_id,
         const unsigned sbx = sb % nsx, sby = (sb / nsx) % nsy, sbz = sb / (nsx * nsy);
         const unsigned lx = (l & 1u) | ((l >> 2) & 2u), ly = ((l >> 1) & 1u) | ((l >> 3) & 2u),
                        lz = ((l >> 2) & 1u) | ((l >> 4) & 2u);
-        const int i = (int)((sbx * 4 + lx) * 4) + (lane & 3);
-        const int j = (int)((sby * 4 + ly) * 4) + ((lane >> 2) & 3);
-        const int k = (int)((sbz * 4 + lz) * 4) + (lane >> 4);
+        int bx, by, bz;
+        brick_log2(a.brick, bx, by, bz);
+        const int i = (int)((sbx * 4 + lx) << bx) + (lane & ((1 << bx) - 1));
+        const int j = (int)((sby * 4 + ly) << by) + ((lane >> bx) & ((1 << by) - 1));
+        const int k = (int)((sbz * 4 + lz) << bz) + (lane >> (bx + by));
         if (i >= a.nx || j >= a.ny || k >= a.nz) return false;
         qid = (long long)i + (long long)a.nx * ((long long)j + (long long)a.ny * k);
         const Q hx = (Q)a.gh[0], hy = (Q)a.gh[1], hz = (Q)a.gh[2];

@@ -166,6 +166,41 @@ def test_batch_autotune_and_schedule(meshes):
     assert info["nselements"] == sum(m.ntris for m in meshes)
     with pytest.raises(S.SsdfError):
         b.set_schedule(99, 1)
+    # the autotuner's per-mesh choices are among its candidates
+    for i in range(len(meshes)):
+        brick, heavy = b.mesh_schedule(i)
+        assert brick in [(4, 4, 4), (8, 4, 2), (4, 8, 2), (2, 4, 8), (8, 8, 1)] and heavy in (4096, 8192, 16384)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_per_mesh_schedules_bit_identical(meshes, seed):
+    """Any per-mesh brick shape and leaf budget gives the same FP32 + fixup results as the default
+    schedule, bit for bit, and the same as each mesh queried alone (budget 1 hands most waves over)."""
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    b = S.Batch(meshes)
+    out = S.DeviceArray.from_host(np.zeros(total, np.float32))
+    tri = S.DeviceArray.from_host(np.zeros(total, np.int32))
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+    want = out.download(), tri.download()
+    rng = np.random.default_rng(seed)
+    shapes = [(4, 4, 4), (8, 4, 2), (2, 4, 8), (8, 8, 1), (16, 4, 1), (1, 1, 64), (64, 1, 1), (2, 32, 1)]
+    for i in range(len(meshes)):
+        b.set_mesh_schedule(i, shapes[int(rng.integers(len(shapes)))], int(rng.choice([-1, 0, 1, 64, 8192])))
+    out.upload(np.zeros(total, np.float32))
+    b.query_grids(S.MODE_SDF, dims, boxes, off, out, tri, flags=S.FLAG_FIXUP)
+    q, err = b.last_stats()
+    assert err == 0
+    assert np.array_equal(out.download(), want[0]) and np.array_equal(tri.download(), want[1])
+    for i, m in enumerate(meshes):
+        so, st = _single_grid(m, dims[i], boxes[i], S.FLAG_FIXUP)
+        if so is not None:
+            N = int(np.prod(dims[i]))
+            assert np.array_equal(want[0][off[i]:off[i] + N], so) and np.array_equal(want[1][off[i]:off[i] + N], st)
+    with pytest.raises(S.SsdfError):
+        b.set_mesh_schedule(0, (4, 4, 2))
+    with pytest.raises(S.SsdfError):
+        b.set_mesh_schedule(0, (3, 3, 7))
 
 
 def test_destroy_after_the_stream_is_gone(meshes):

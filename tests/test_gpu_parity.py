@@ -516,3 +516,54 @@ def test_fixup_slivers_follow_the_fp64_routine(oracle, leaves):
     assert err == 0 and q >= x.size
     assert np.array_equal(t.download(), tri)
     bars.assert_fixup_distances(out.download(), np.sqrt(d2), "sliver traps")
+
+
+@pytest.mark.parametrize("brick", [(8, 4, 2), (2, 4, 8), (8, 8, 1), (1, 64, 1), (64, 1, 1)])
+def test_mesh_brick_schedules_bit_identical(brick):
+    """A handle's grid waves of any 64-node brick give the same FP32 + fixup results, bit for bit, as
+    the default 4 x 4 x 4 bricks, on a ragged grid (partial bricks on every face)."""
+    m = mg.torus(0.6, 0.2, 80, 32, sigma=0.004, seed=3)
+    rm = S.ResidentMesh.from_mesh(m)
+    n, lo, hi = (37, 29, 21), (-1, -1, -0.45), (1, 1, 0.45)
+    N = n[0] * n[1] * n[2]
+    res = []
+    for sched in ((4, 4, 4), brick):
+        rm.set_schedule(sched)
+        o = S.DeviceArray.from_host(np.full(N, 7.0, np.float32))
+        t = S.DeviceArray.from_host(np.full(N, -9, np.int32))
+        rm.query_grid(S.MODE_SDF, n, lo, hi, o, t, flags=S.FLAG_FIXUP)
+        q, err = rm.last_stats()
+        assert err == 0
+        res.append((o.download(), t.download()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert np.all(res[1][1] >= 0)  # every node written
+
+
+def test_cell_validate_environment(capfd, monkeypatch):
+    """SSDF_CELL_VALIDATE=1 (serial/ssdf.hpp:991-1010): the C symbols check their result against the
+    exact FP64 argmin of the same inputs, print the maximum and the norm of the relative differences
+    to stderr when the maximum exceeds 1e-5, and return the engine's own result."""
+    m = mg.uv_sphere(0.5, 24, 12)
+    v = np.stack([m.sx, m.sy, m.sz], 1).astype(np.float64)
+    cen = ((v[m.s0] + v[m.s1] + v[m.s2]) / 3).astype(np.float32)  # on the faces: FP32 error dominates
+    x, y, z = (np.ascontiguousarray(cen[:, d]) for d in range(3))
+    base = np.full(x.size, 1e9, np.float32)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, base)
+    capfd.readouterr()
+    monkeypatch.setenv("SSDF_CELL_VALIDATE", "1")
+    out = np.full(x.size, 1e9, np.float32)
+    S.edf(x, y, z, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    err = capfd.readouterr().err
+    assert np.array_equal(out, base)
+    assert "EDF cell-list validation max rel diff:" in err and "EDF cell-list validation rel diff norm:" in err
+    # far from the surface the FP32 result agrees to 1e-5: nothing printed
+    far = [np.ascontiguousarray(a * np.float32(3.0)) for a in (x, y, z)]
+    out = np.full(x.size, 1e9, np.float32)
+    S.edf(*far, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, out)
+    assert "validation" not in capfd.readouterr().err
+    # FP64 inputs: the exact path checked against itself, silent and unchanged
+    md = m.as_dtype(np.float64)
+    o64 = np.full(x.size, 1e9)
+    S.edf(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64), md.s0, md.s1, md.s2, md.sx, md.sy, md.sz,
+          o64)
+    assert "validation" not in capfd.readouterr().err
