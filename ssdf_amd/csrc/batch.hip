@@ -19,6 +19,11 @@
 
 using namespace ssdf_amd;
 
+// A mesh's share of the batch's connectivity arena: 3 corners of kConnBytesPerCorner bytes per
+// triangle, rounded up to 64 so that every mesh's 8-byte edge keys stay aligned.
+constexpr size_t kConnStride = 64;
+static_assert(kConnStride >= 3 * kConnBytesPerCorner, "a triangle's corners fit its stride");
+
 struct ssdf_batch {
     int nm = 0;
     bool with_normals = false;
@@ -31,6 +36,7 @@ struct ssdf_batch {
     TriP *tris = nullptr;
     MeshMeta *metas = nullptr;
     double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
+    char *conn = nullptr;  // per mesh, the sorted connectivity of its normals (kConnStride bytes a triangle)
     char *ws = nullptr;  // build workspace of lane 0
     size_t ws_cap = 0;
     // Build lanes: meshes are built concurrently on up to kLanes streams (lane 0: the caller's),
@@ -71,7 +77,7 @@ void destroy_batch(ssdf_batch *b) {
     const hipStream_t st = nullptr;
     for (void *p : {(void *)b->s0, (void *)b->s1, (void *)b->s2, (void *)b->sx, (void *)b->sy, (void *)b->sz,
                     (void *)b->nodes, (void *)b->tris, (void *)b->metas, (void *)b->face_n, (void *)b->edge_n,
-                    (void *)b->vertex_n, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
+                    (void *)b->vertex_n, (void *)b->conn, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
                     (void *)b->fix_list, (void *)b->cost})
         dev_free(p, st);
     for (int l = 1; l < ssdf_batch::kLanes; l++) dev_free(b->lane_ws[l], st);
@@ -430,6 +436,7 @@ int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t
         if (int rc = arena(&b->face_n, 3 * (size_t)T, b)) return rc;
         if (int rc = arena(&b->edge_n, 9 * (size_t)T, b)) return rc;
         if (int rc = arena(&b->vertex_n, 3 * (size_t)V, b)) return rc;
+        if (int rc = arena(&b->conn, kConnStride * (size_t)T, b)) return rc;
     }
     const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipStream_t st = b->stream;
@@ -461,6 +468,7 @@ int ssdf_batch_create(int nmeshes, const ptrdiff_t *tri_offsets, const ptrdiff_t
             m.face_n = b->face_n + 3 * b->tri_off[i];
             m.edge_n = b->edge_n + 9 * b->tri_off[i];
             m.vertex_n = b->vertex_n + 3 * b->vert_off[i];
+            m.conn = b->conn + kConnStride * b->tri_off[i];
         }
         m.stream = st;
     }

@@ -431,7 +431,7 @@ size_t mesh_bytes_bound(ptrdiff_t nsel, ptrdiff_t nsp, int dtype, bool with_norm
     b += sizeof(Node) * ni + sizeof(TriP) * n + sizeof(MeshMeta);   // LBVH
     size_t ws = lbvh_ws_bytes(n);
     if (with_normals) {
-        b += sizeof(double) * (12 * n + 3 * V);
+        b += sizeof(double) * (12 * n + 3 * V) + kConnBytesPerCorner * 3 * n;
         ws = std::max(ws, normals_ws_bytes(n));
     }
     return b + ws;
@@ -548,6 +548,7 @@ __global__ __launch_bounds__(256) void k_face_normals(const int *__restrict__ s0
         const double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
         const double cosine = (l1 > 0 && l2 > 0) ? (e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2]) / (l1 * l2) : 1.0;
         angle[(size_t)3 * t + v] = acos(fmax(-1.0, fmin(1.0, cosine)));
+        if (!vkeys) continue;  // the connectivity's sorted corners are already known
         vkeys[(size_t)3 * t + v] = (uint32_t)iv[v];
         vvals[(size_t)3 * t + v] = (uint32_t)(3 * t + v);
         const uint64_t ea = (uint64_t)iv[v], eb = (uint64_t)iv[(v + 1) % 3];
@@ -621,13 +622,23 @@ __global__ void k_meta_zero_edges(MeshMeta *meta) {
 
 int build_normals(DevMesh &m, hipStream_t stream) {
     const size_t n = (size_t)m.nsel, V = (size_t)m.nsp;
+    const size_t e = 3 * n;
     if (!m.face_n) {
         SSDF_HIP_TRY(dev_alloc((void **)&m.face_n, sizeof(double) * 3 * n, stream));
         SSDF_HIP_TRY(dev_alloc((void **)&m.edge_n, sizeof(double) * 9 * n, stream));
         SSDF_HIP_TRY(dev_alloc((void **)&m.vertex_n, sizeof(double) * 3 * V, stream));
         m.device_bytes += sizeof(double) * (12 * n + 3 * V);
     }
-    const size_t e = 3 * n;
+    if (!m.conn) {
+        SSDF_HIP_TRY(dev_alloc((void **)&m.conn, kConnBytesPerCorner * e, stream));
+        m.device_bytes += kConnBytesPerCorner * e;
+    }
+    // the corners sorted by vertex id and by edge key: they depend on the connectivity alone, which
+    // a handle never changes, so only the first build sorts them
+    uint32_t *cvk = (uint32_t *)m.conn, *cvv = cvk + e;
+    uint64_t *cek = (uint64_t *)(m.conn + 8 * e);
+    uint32_t *cev = (uint32_t *)(m.conn + 16 * e);
+    const bool sort = !m.conn_ready;
     // workspace: angle 8 B, vertex keys/vals 2 x 2 x 4 B, edge keys 2 x 8 B, edge vals 2 x 4 B per
     // corner, radix scratch
     const size_t off_angle = 0, off_vk = off_angle + 8 * e, off_vv = off_vk + 8 * e, off_ek = off_vv + 8 * e,
@@ -639,26 +650,37 @@ int build_normals(DevMesh &m, hipStream_t stream) {
     uint32_t *evals = (uint32_t *)(m.ws + off_ev), *scratch = (uint32_t *)(m.ws + off_scr);
     hipLaunchKernelGGL(k_meta_zero_edges, dim3(1), dim3(1), 0, stream, m.meta);
     hipLaunchKernelGGL(k_fill_f64, dim3(nblk(3 * V, 256)), dim3(256), 0, stream, m.vertex_n, 3 * V, 0.0);
+    uint32_t *kv = sort ? vkeys : nullptr, *kvv = sort ? vvals : nullptr, *kev = sort ? evals : nullptr;
+    uint64_t *kek = sort ? ekeys : nullptr;
     if (m.dtype == 0)
         hipLaunchKernelGGL(k_face_normals<float>, dim3(nblk(n, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
                            (const float *)m.sx, (const float *)m.sy, (const float *)m.sz, (int)n, (long long)V,
-                           m.face_n, m.edge_n, angle, vkeys, vvals, ekeys, evals);
+                           m.face_n, m.edge_n, angle, kv, kvv, kek, kev);
     else
         hipLaunchKernelGGL(k_face_normals<double>, dim3(nblk(n, 256)), dim3(256), 0, stream, m.s0, m.s1, m.s2,
                            (const double *)m.sx, (const double *)m.sy, (const double *)m.sz, (int)n, (long long)V,
-                           m.face_n, m.edge_n, angle, vkeys, vvals, ekeys, evals);
-    int vbits = 1;
-    while (vbits < 32 && ((size_t)1 << vbits) < V) vbits++;
-    int ebits = 1;
-    const unsigned long long maxkey = (unsigned long long)V * (unsigned long long)V;
-    while (ebits < 64 && (1ull << ebits) < maxkey) ebits++;
-    uint32_t *svk = nullptr, *svv = nullptr, *sev = nullptr;
-    uint64_t *sek = nullptr;
-    SSDF_HIP_TRY(radix_sort_pairs<uint32_t>(vkeys, vkeys + e, vvals, vvals + e, e, vbits, scratch, stream, &svk, &svv));
-    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, svk, svv, e, m.face_n, angle,
+                           m.face_n, m.edge_n, angle, kv, kvv, kek, kev);
+    if (sort) {
+        int vbits = 1;
+        while (vbits < 32 && ((size_t)1 << vbits) < V) vbits++;
+        int ebits = 1;
+        const unsigned long long maxkey = (unsigned long long)V * (unsigned long long)V;
+        while (ebits < 64 && (1ull << ebits) < maxkey) ebits++;
+        uint32_t *svk = nullptr, *svv = nullptr, *sev = nullptr;
+        uint64_t *sek = nullptr;
+        SSDF_HIP_TRY(
+            radix_sort_pairs<uint32_t>(vkeys, vkeys + e, vvals, vvals + e, e, vbits, scratch, stream, &svk, &svv));
+        SSDF_HIP_TRY(
+            radix_sort_pairs<uint64_t>(ekeys, ekeys + e, evals, evals + e, e, ebits, scratch, stream, &sek, &sev));
+        SSDF_HIP_TRY(hipMemcpyAsync(cvk, svk, sizeof(uint32_t) * e, hipMemcpyDeviceToDevice, stream));
+        SSDF_HIP_TRY(hipMemcpyAsync(cvv, svv, sizeof(uint32_t) * e, hipMemcpyDeviceToDevice, stream));
+        SSDF_HIP_TRY(hipMemcpyAsync(cek, sek, sizeof(uint64_t) * e, hipMemcpyDeviceToDevice, stream));
+        SSDF_HIP_TRY(hipMemcpyAsync(cev, sev, sizeof(uint32_t) * e, hipMemcpyDeviceToDevice, stream));
+        m.conn_ready = true;
+    }
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, cvk, cvv, e, m.face_n, angle,
                        m.vertex_n);
-    SSDF_HIP_TRY(radix_sort_pairs<uint64_t>(ekeys, ekeys + e, evals, evals + e, e, ebits, scratch, stream, &sek, &sev));
-    hipLaunchKernelGGL(k_edge_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, sek, sev, e, m.s0, m.s1, m.s2,
+    hipLaunchKernelGGL(k_edge_normals, dim3(nblk(e, 256)), dim3(256), 0, stream, cek, cev, e, m.s0, m.s1, m.s2,
                        m.face_n, m.edge_n, m.meta);
     SSDF_HIP_TRY(hipGetLastError());
     m.has_normals = true;
@@ -680,6 +702,7 @@ void free_mesh(DevMesh &m, hipStream_t st) {
     dev_free(m.face_n, st);
     dev_free(m.edge_n, st);
     dev_free(m.vertex_n, st);
+    dev_free(m.conn, st);
     dev_free(m.gwn, st);
     dev_free(m.ws, st);
     m = DevMesh{};

@@ -19,6 +19,8 @@ struct MeshMeta {
     int flipped_edges;  // manifold edges whose two faces traverse it in the same direction
 };
 
+constexpr size_t kConnBytesPerCorner = 4 + 4 + 8 + 4;
+
 struct DevMesh {
     int dtype = 0;  // 0: float vertices, 1: double vertices
     ptrdiff_t nsel = 0, nsp = 0;
@@ -34,6 +36,11 @@ struct DevMesh {
     // pseudonormals (FP64): face 3T, edge 9T, vertex 3V
     double *face_n = nullptr, *edge_n = nullptr, *vertex_n = nullptr;
     bool has_normals = false;
+    // the normals' view of the connectivity, sorted once by the first normals build (a handle's
+    // triangles never change, only its vertex positions): the 3T corners by vertex id (keys,
+    // corner ids) and by edge key (keys, corner ids), kConnBytesPerCorner bytes per corner
+    char *conn = nullptr;
+    bool conn_ready = false;
     // winding-number far field (nsel - 1 records), built on first use, stale after a rebuild
     GwnNode *gwn = nullptr;
     bool has_gwn = false;

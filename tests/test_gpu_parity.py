@@ -387,6 +387,49 @@ def test_moving_mesh_update_vertices(oracle):
     assert rm.info()["open_edges"] == 0
 
 
+def _odd_topology(seed):
+    """A noisy torus with 2% of its faces deleted (open edges), one face repeated (three faces on its
+    edges: non-manifold) and one face reversed (flipped edges)."""
+    m = mg.torus(0.6, 0.2, 60, 24, sigma=0.004, seed=seed)
+    f = np.stack([m.s0, m.s1, m.s2], 1)
+    rng = np.random.default_rng(seed)
+    keep = rng.uniform(size=len(f)) >= 0.02
+    f = f[keep]
+    f = np.concatenate([f, f[5:6]])            # repeated face
+    f[17] = f[17][::-1]                        # reversed face
+    v = np.stack([m.sx, m.sy, m.sz], 1)
+    return mg._mesh(v.astype(np.float64), f, "odd_torus")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rebuild_normals_equal_a_fresh_build(dtype):
+    """A handle sorts its corners by vertex and by edge once (the connectivity never changes) and
+    reuses them when it rebuilds after its vertices move: every rebuilt handle answers SDF queries
+    bit for bit like a handle built fresh at the same positions, with the same open and flipped
+    edge counts, on a mesh with holes, a non-manifold edge and a reversed face."""
+    m0 = _odd_topology(1)
+    rng = np.random.default_rng(2)
+    n, lo, hi = (20, 20, 10), (-1, -1, -0.4), (1, 1, 0.4)
+    N = n[0] * n[1] * n[2]
+    m0 = m0.as_dtype(dtype)
+    rm = S.ResidentMesh.from_mesh(m0, dtype=dtype)
+    for step in range(3):
+        jit = (0.004 * rng.normal(size=(m0.sx.size, 3))).astype(dtype)
+        sx, sy, sz = (np.ascontiguousarray(a + jit[:, d]) for d, a in enumerate((m0.sx, m0.sy, m0.sz)))
+        rm.update_vertices(sx, sy, sz)
+        moved = mg.Mesh(sx, sy, sz, m0.s0, m0.s1, m0.s2, "moved")
+        fresh = S.ResidentMesh.from_mesh(moved, dtype=dtype)
+        res = []
+        for h in (rm, fresh):
+            o = S.DeviceArray(N, dtype)
+            t = S.DeviceArray(N, np.int32)
+            h.query_grid(S.MODE_SDF, n, lo, hi, o, t, flags=S.FLAG_FIXUP)
+            res.append((o.download(), t.download(), h.info()["open_edges"], h.info()["flipped_edges"]))
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), step
+        assert res[0][2:] == res[1][2:] and res[0][2] > 0 and res[0][3] > 0, res[0][2:]
+        fresh.close()
+
+
 def test_fixup_queue_many_ties(oracle):
     """Points on the axis of a UV sphere are equidistant from every triangle of a latitude ring (48
     exact ties, beyond the 24-entry candidate list): they take the fixup queue, whose FP64 search must
