@@ -223,6 +223,13 @@ int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const i
     }
     ssdf_mesh *h = new (std::nothrow) ssdf_mesh();
     if (!h) return SSDF_ERR_OOM;
+    // the ordering event exists from the start, so that ordering a call (stream_after) never
+    // allocates: the allocation-free query orders itself too
+    if (hipEventCreateWithFlags(&h->order_ev, hipEventDisableTiming) != hipSuccess) {
+        delete h;
+        set_error("ssdf_mesh_create: hipEventCreateWithFlags failed");
+        return SSDF_ERR_HIP;
+    }
     int rc = dtype == 0 ? upload_mesh<float>(h->m, nsel, s0, s1, s2, nsp, (const float *)sx, (const float *)sy,
                                              (const float *)sz, device_ptrs, stream)
                         : upload_mesh<double>(h->m, nsel, s0, s1, s2, nsp, (const double *)sx, (const double *)sy,
@@ -232,6 +239,7 @@ int create_mesh(int dtype, ptrdiff_t nsel, const int *s0, const int *s1, const i
     if (!rc && nsel > 0) rc = sync_meta(h->m, stream);
     if (rc) {
         free_mesh(h->m, stream);
+        (void)hipEventDestroy(h->order_ev);
         delete h;
         return rc;
     }
@@ -666,13 +674,20 @@ int server_mesh_create(const int *s0, const int *s1, const int *s2, ptrdiff_t ns
 // only its read-only tree and may run concurrently on different streams.  The winding-number tree
 // that SSDF_FLAG_SIGN_GWN / AUTO need is built here (allocating, synchronous) when may_build_gwn,
 // else its absence is an error.
-int ws_query_points(ssdf_mesh_t h, void *ws, bool may_build_gwn, int mode, ptrdiff_t n, const void *x,
+int ws_query_points(ssdf_mesh_t h, void *ws, bool handle_call, int mode, ptrdiff_t n, const void *x,
                     const void *y, const void *z, void *out_d, int *out_tri, void *cx, void *cy, void *cz,
                     unsigned flags, hipStream_t stream) {
     if (n <= 0 || h->m.nsel <= 0) return SSDF_OK;
     if (int rc = check_points_args(mode, x, y, z, out_d, cx, cy, cz)) return rc;
+    if (handle_call) {
+        // ordered like the handle's other calls: after its builds and its last call, and recorded
+        // as its last call, which its rebuilds and its destroy wait for
+        if (int rc = order_after_handle(h, stream)) return rc;
+        h->queried = true;
+        h->last_stream = stream;
+    }
     if (mode == SSDF_MODE_SDF && gwn_sign_wanted(h->m, flags) && !h->m.has_gwn) {
-        if (!may_build_gwn) {
+        if (handle_call) {
             set_error("the mesh's winding-number tree is not built (run one allocating winding-number or "
                       "signed query on it first)");
             return SSDF_ERR_INVALID;

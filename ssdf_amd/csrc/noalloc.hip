@@ -68,6 +68,6 @@ extern "C" int ssdf_mesh_query_points_noalloc(ssdf_mesh_t mesh, int mode, ptrdif
     if (npoints <= 0) return SSDF_OK;
     void *ws = nullptr;
     if (int rc = noalloc_scratch(&ws)) return rc;
-    return ws_query_points(mesh, ws, false, mode, npoints, x, y, z, out_d, out_tri, out_cx, out_cy, out_cz, flags,
+    return ws_query_points(mesh, ws, true, mode, npoints, x, y, z, out_d, out_tri, out_cx, out_cy, out_cz, flags,
                            (hipStream_t)stream);
 }

@@ -221,8 +221,12 @@ inline QueryWs carve_ws(void *base, size_t nq, bool sorted) {
     return w;
 }
 
-// A point query on a resident mesh in the caller's workspace `ws` (capi.hip).
-int ws_query_points(ssdf_mesh_t h, void *ws, bool may_build_gwn, int mode, ptrdiff_t n, const void *x,
+// A point query on a resident mesh in the caller's workspace `ws` (capi.hip).  handle_call: a call
+// of the handle's owner (the allocation-free entry point), ordered after the handle's builds and
+// its last call and recorded as its last call; it allocates nothing, so the winding-number tree a
+// signed query may need must exist.  Otherwise a serving request: read-only, ordered by the server,
+// and it may build that tree (synchronously, on the mesh's stream).
+int ws_query_points(ssdf_mesh_t h, void *ws, bool handle_call, int mode, ptrdiff_t n, const void *x,
                     const void *y, const void *z, void *out_d, int *out_tri, void *cx, void *cy, void *cz,
                     unsigned flags, hipStream_t stream);
 

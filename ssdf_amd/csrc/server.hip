@@ -206,7 +206,7 @@ bool admit_one(ssdf_server *s, std::unique_lock<std::mutex> &lk) {
                 he = hipMemcpyAsync(dt, r->out_tri, 4 * n, hipMemcpyHostToDevice, st);
             if (he != hipSuccess) rc = SSDF_ERR_HIP;
             if (!rc)
-                rc = ws_query_points(e.handle, d + request_io_bytes(r), true, r->mode, r->n, dx, dy, dz, dd,
+                rc = ws_query_points(e.handle, d + request_io_bytes(r), false, r->mode, r->n, dx, dy, dz, dd,
                                      r->out_tri ? dt : nullptr, nullptr, nullptr, nullptr, r->flags, st);
             if (!rc) {
                 he = hipMemcpyAsync(r->out_d, dd, 4 * n, hipMemcpyDeviceToHost, st);

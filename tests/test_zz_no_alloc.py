@@ -102,3 +102,28 @@ def test_noalloc_rejects_oversize(gpu_lib, torus):
     ref = _run(rm, S.MODE_EDF, pts, S.FLAG_FIXUP, noalloc=False)
     got = _run(rm, S.MODE_EDF, pts, S.FLAG_FIXUP, noalloc=True)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+
+
+def test_noalloc_is_ordered_like_the_handles_calls(gpu_lib):
+    """The allocation-free query waits for the handle's last build and last call whatever streams
+    they ran on (with no allocation: the ordering event exists from the handle's creation): the
+    vertices move on one stream and the query, issued at once on another, answers for the moved
+    mesh, bit for bit as the allocating entry point does after a synchronization; a rebuild issued
+    right after on the first stream waits for the query, whose results stay intact."""
+    m0 = mg.torus(0.6, 0.2, 120, 50, sigma=0.004, seed=7)
+    m1 = mg.torus(0.6, 0.2, 120, 50, sigma=0.004, seed=8)
+    sa, sb = S.Stream(), S.Stream()
+    rm = S.ResidentMesh.from_mesh(m0, stream=sa)
+    pts = _points(m1, 200_000, seed=5)
+    x, y, z = (S.DeviceArray.from_host(p) for p in pts)
+    d = S.DeviceArray(pts[0].size, np.float32)
+    t = S.DeviceArray(pts[0].size, np.int32)
+    rm.update_vertices(m1.sx, m1.sy, m1.sz, stream=sa)
+    rm.query_points_noalloc(S.MODE_SDF, x, y, z, d, t, flags=S.FLAG_FIXUP, stream=sb)
+    rm.update_vertices(m0.sx, m0.sy, m0.sz, stream=sa)  # waits for the query on sb
+    sa.synchronize()
+    sb.synchronize()
+    got = (d.download(), t.download())
+    ref_rm = S.ResidentMesh.from_mesh(m1)
+    ref = _run(ref_rm, S.MODE_SDF, pts, S.FLAG_FIXUP, noalloc=False)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
