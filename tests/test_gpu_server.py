@@ -110,3 +110,47 @@ def test_rejects_what_can_never_fit(pool):
     with pytest.raises(S.SsdfError):
         srv.submit(5, S.MODE_SDF, x[:10], x[:10], x[:10], np.zeros(10, np.float32))
     srv.close()
+
+
+def test_submitters_on_several_threads(pool):
+    """Requests submitted and waited for from four host threads at once (the C calls release the
+    interpreter lock, so the server sees concurrent submits): every result equals the direct
+    query's, and the statistics count every request once."""
+    import threading
+    import time
+
+    srv = S.Server(8 << 30)
+    for i, m in enumerate(pool):
+        srv.add_mesh(i, m)
+    per_thread = [_requests(pool, 8, 100 + k) for k in range(4)]
+    results = [None] * 4
+    errors = []
+
+    def worker(k):
+        try:
+            mine = []
+            for mid, mode, flags, (x, y, z), init in per_thread[k]:
+                out = init.copy()
+                tri = np.full(x.size, -1, np.int32)
+                mine.append((srv.submit(mid, mode, x, y, z, out, tri, flags=flags), out, tri))
+            for t, _, _ in mine:
+                while not srv.poll(t):
+                    time.sleep(0.0005)
+                assert srv.wait(t) >= 0
+            results[k] = [(out, tri) for _, out, tri in mine]
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for k in range(4):
+        for (mid, mode, flags, (x, y, z), init), (out, tri) in zip(per_thread[k], results[k]):
+            d, t = _direct(pool[mid], mode, x, y, z, init, flags)
+            assert np.array_equal(out, d) and np.array_equal(tri, t)
+    st = srv.stats()
+    assert st["served"] == 32 and st["errors"] == 0 and st["pending"] == 0 and st["inflight"] == 0
+    srv.close()
