@@ -191,9 +191,11 @@ __device__ __forceinline__ float solid_angle(float qx, float qy, float qz, const
     const float ax = A[0] - qx, ay = A[1] - qy, az = A[2] - qz;
     const float bx = B[0] - qx, by = B[1] - qy, bz = B[2] - qz;
     const float cx = C[0] - qx, cy = C[1] - qy, cz = C[2] - qz;
-    const float la = sqrtf(ax * ax + ay * ay + az * az);
-    const float lb = sqrtf(bx * bx + by * by + bz * bz);
-    const float lc = sqrtf(cx * cx + cy * cy + cz * cz);
+    // the hardware square root (1 ulp), not the correctly rounded sequence: the winding number is an
+    // approximation to 0.03 anyway
+    const float la = __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az);
+    const float lb = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
+    const float lc = __builtin_amdgcn_sqrtf(cx * cx + cy * cy + cz * cz);
     const float det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
     const float den = la * lb * lc + (ax * bx + ay * by + az * bz) * lc + (ax * cx + ay * cy + az * cz) * lb +
                       (bx * cx + by * cy + bz * cz) * la;
@@ -226,7 +228,9 @@ __global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, cons
             const float d2 = dx * dx + dy * dy + dz * dz;
             const bool near = valid && !(d2 > beta2 * r2);
             if (__ballot(near) == 0) {
-                const float inv = 1.0f / (d2 * sqrtf(d2));
+                // |c - q|^-3 from the hardware reciprocal square root (1 ulp; d2 > 0 here)
+                const float rs = __builtin_amdgcn_rsqf(d2);
+                const float inv = rs * rs * rs;
                 acc += (n[0] * dx + n[1] * dy + n[2] * dz) * inv;
             } else if (child < 0) {
                 const int k0 = leaf_first(child), nl = leaf_count(child);
