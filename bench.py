@@ -42,7 +42,7 @@ METRIC = "Mqueries/s @256^3×200k tris (FP32+fixup vs plain); served p50/p99 lat
 FLOPS_BOX, FLOPS_TRI, FLOPS_SIGN = 8, 81, 8
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-HOST_THREADS_MAX = 16  # the GPU box's CPU share
+HOST_THREADS_MAX = 16  # host threads this process may use per GPU (the host's CPU share)
 
 
 def log(*a):
