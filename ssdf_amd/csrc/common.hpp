@@ -217,14 +217,28 @@ __host__ __device__ __forceinline__ float box_d2_fast(const float px, const floa
 // few ulps of the coordinates, with no per-lane branches or divisions (every lane of a wave runs the
 // same ~65 instructions); the exact FP64 routine re-decides near-ties.  Degenerate triangles carry
 // NaN side normals, so they take the segment branch.
-__host__ __device__ __forceinline__ float tri_d2_fast(const float px, const float py, const float pz, const TriP &t) {
-    const float pa0 = px - t.f.a[0], pa1 = py - t.f.a[1], pa2 = pz - t.f.a[2];
+//
+// It is split in two so that a wave can stop after the first part: tri_plane gives p - a and the
+// signed plane distance h = n.(p - a) (unit normal n; 0 on triangles flagged degenerate), and |h|
+// bounds the distance from below (plane_bound below); tri_d2_rest finishes from there.
+struct TriPlane {
+    float pa0, pa1, pa2, h;
+};
+__host__ __device__ __forceinline__ TriPlane tri_plane(const float px, const float py, const float pz, const TriP &t) {
+    TriPlane r;
+    r.pa0 = px - t.f.a[0], r.pa1 = py - t.f.a[1], r.pa2 = pz - t.f.a[2];
+    r.h = __builtin_fmaf(t.nh[0], r.pa0, __builtin_fmaf(t.nh[1], r.pa1, t.nh[2] * r.pa2));
+    return r;
+}
+
+__host__ __device__ __forceinline__ float tri_d2_rest(const float px, const float py, const float pz, const TriP &t,
+                                                      const TriPlane &pl) {
+    const float pa0 = pl.pa0, pa1 = pl.pa1, pa2 = pl.pa2, h = pl.h;
     const float pb0 = px - t.f.b[0], pb1 = py - t.f.b[1], pb2 = pz - t.f.b[2];
     const float pc0 = px - t.f.c[0], pc1 = py - t.f.c[1], pc2 = pz - t.f.c[2];
     const float s0 = __builtin_fmaf(t.m[0], pa0, __builtin_fmaf(t.m[1], pa1, t.m[2] * pa2));
     const float s1 = __builtin_fmaf(t.m[3], pb0, __builtin_fmaf(t.m[4], pb1, t.m[5] * pb2));
     const float s2 = __builtin_fmaf(t.m[6], pc0, __builtin_fmaf(t.m[7], pc1, t.m[8] * pc2));
-    const float h = __builtin_fmaf(t.nh[0], pa0, __builtin_fmaf(t.nh[1], pa1, t.nh[2] * pa2));
     // edge segments: t = clamp(e.p / |e|^2, 0, 1), r = p - t e
     float u0 = __builtin_fmaf(t.e[0], pa0, __builtin_fmaf(t.e[1], pa1, t.e[2] * pa2)) * t.inv[0];
     float u1 = __builtin_fmaf(t.e[3], pb0, __builtin_fmaf(t.e[4], pb1, t.e[5] * pb2)) * t.inv[1];
@@ -246,6 +260,21 @@ __host__ __device__ __forceinline__ float tri_d2_fast(const float px, const floa
     const int msk = -(int)inside;
     const int dp = __builtin_bit_cast(int, h * h), ds = __builtin_bit_cast(int, fminf(q0, fminf(q1, q2)));
     return __builtin_bit_cast(float, (dp & msk) | (ds & ~msk));
+}
+
+__host__ __device__ __forceinline__ float tri_d2_fast(const float px, const float py, const float pz, const TriP &t) {
+    return tri_d2_rest(px, py, pz, t, tri_plane(px, py, pz, t));
+}
+
+// Plane bound of a pruning bound d2b (a squared distance, strict or inclusive): every triangle whose
+// tri_d2_fast value is at most d2b has |h| <= plane_bound(r, babs), where r is sqrt(d2b) to 1 ulp
+// or above and babs >= 2^-18 max(L, |p|) (L: the largest coordinate magnitude of the mesh).  Why:
+// sqrt(d_fast) is within 2^-19 max of the exact distance d, |h_exact| <= d, and the FP32 h is
+// within 18 * 2^-24 max of h_exact (the roundings of p - a and of the unit normal, three in the
+// FMA chain, over |p - a| <= 2 sqrt(3) max); 2 babs covers those terms, and the factor on r its
+// rounding.  r = NaN or -inf (no query, or one handed over) gives a bound that no |h| passes.
+__host__ __device__ __forceinline__ float plane_bound(const float r, const float babs) {
+    return __builtin_fmaf(r, 1.0f + 0x1p-22f, 2.0f * babs);
 }
 
 // Fills a TriP from FP32 vertices: the constants are evaluated in FP64 from the FP32 values and

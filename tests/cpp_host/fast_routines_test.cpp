@@ -1,6 +1,7 @@
-// Host build of the engine's FP32 pruning routines (box_d2_fast, tri_d2_fast, make_trip from
-// ssdf_amd/csrc/common.hpp): reads (point, triangle) cases on stdin, writes the FP32 squared
-// distances on stdout; tests/test_fast_routines.py checks them against the FP64 oracle.
+// Host build of the engine's FP32 pruning routines (box_d2_fast, tri_d2_fast, tri_plane, make_trip
+// from ssdf_amd/csrc/common.hpp): reads (point, triangle) cases on stdin, writes the FP32 squared
+// distances, the degenerate flag and the plane distance on stdout; tests/test_fast_routines.py
+// checks them against the FP64 oracle.
 #include <cstdio>
 
 #include "common.hpp"
@@ -21,7 +22,8 @@ int main() {
             mx[k] = std::fmax(a[k], std::fmax(b[k], c[k]));
         }
         const float bx = ssdf_amd::box_d2_fast(p[0], p[1], p[2], mn, mx);
-        std::printf("%a %a %d\n", d, bx, t.f.flags);
+        const float h = ssdf_amd::tri_plane(p[0], p[1], p[2], t).h;
+        std::printf("%a %a %d %a\n", d, bx, t.f.flags, h);
     }
     return 0;
 }

@@ -37,6 +37,34 @@ def run(exe, cases):
             np.array([int(o[2]) for o in out]))
 
 
+def run_plane(exe, cases):
+    """FP32 plane distances h = n.(p - a) of the cases (tri_plane)."""
+    lines = [str(len(cases))]
+    for cs in cases:
+        lines.append(" ".join(float(v).hex() for x in cs for v in x))
+    r = subprocess.run([str(exe)], input="\n".join(lines), capture_output=True, text=True, check=True)
+    return np.array([float.fromhex(l.split()[3]) for l in r.stdout.strip().splitlines()])
+
+
+def plane_bound(d2b, babs):
+    """plane_bound (common.hpp) of a bound d2b, in FP32 as the kernels evaluate it."""
+    r = np.sqrt(np.float32(d2b)).astype(np.float32)
+    return (r * np.float32(1.0 + 2.0 ** -22) + np.float32(2.0) * np.float32(babs)).astype(np.float32)
+
+
+def check_plane_skip(exe, cases, d_fast):
+    """The wave-level plane skip is exact: a triangle whose FP32 distance is at most a lane's bound
+    is never skipped for that lane, i.e. |h| <= plane_bound(d_fast) with babs = 2^-18 max(L, |p|),
+    the smallest babs a kernel uses (bounds above d_fast only widen it).  Also checks that the
+    bound is not vacuous: |h| is within 2^-18 L of the distance on non-degenerate triangles."""
+    h = run_plane(exe, cases)
+    L = np.array([max(1.0, float(np.abs(np.concatenate(cs[1:])).max())) for cs in cases])
+    P = np.array([float(np.abs(cs[0]).max()) for cs in cases])
+    babs = (2.0 ** -18 * np.maximum(L, P)).astype(np.float32)
+    assert np.all(np.abs(h) <= plane_bound(d_fast, babs)), "a triangle within the bound would be skipped"
+    return h, L
+
+
 def test_tri_d2_fast_accuracy(exe, oracle):
     cases = _random_cases(4000, 21, np.float32)
     d, bx, fl = run(exe, cases)
@@ -46,6 +74,13 @@ def test_tri_d2_fast_accuracy(exe, oracle):
     assert np.all(err <= 2.0 ** -19 * L), err.max()  # half the fixup band's absolute term
     assert np.all(np.sqrt(bx) <= np.sqrt(ref) + 2.0 ** -19 * L)  # conservative pruning
     assert np.all(fl == 0)
+    h, _ = check_plane_skip(exe, cases, d)
+    # the plane distance is a lower bound of the distance (to the FP32 rounding of h), and equals it
+    # where the projection falls inside the triangle
+    over = np.abs(h) - np.sqrt(ref)
+    assert np.all(over <= 2.0 ** -19 * L), over.max()
+    face = np.abs(over) <= 2.0 ** -19 * L
+    assert 0.05 < face.mean() < 0.95
 
 
 def test_tri_d2_fast_degenerate(exe, oracle):
@@ -60,6 +95,8 @@ def test_tri_d2_fast_degenerate(exe, oracle):
     cases.append(tuple(np.asarray(v, np.float32) for v in ((0.3, 0.2, 0.1), (0, 0, 0), (0, 0, 0), (0, 0, 0))))
     d, bx, fl = run(exe, cases)
     assert np.all(np.isfinite(d))
+    h, _ = check_plane_skip(exe, cases, d)
+    assert np.all(h[fl == 1] == 0)  # no normal: never skipped by the plane test
     for (p, a, b, c), dd in zip(cases, d):
         p, a, b, c = (x.astype(np.float64) for x in (p, a, b, c))
         seg = []
@@ -131,6 +168,7 @@ def test_tri_d2_fast_slivers(exe, oracle):
     L = np.array([max(1.0, float(np.abs(np.concatenate(cs)).max())) for cs in cases])
     err = np.abs(np.sqrt(d) - np.sqrt(exact))
     assert np.all(err <= 2.0 ** -19 * L), (err / L).max()
+    check_plane_skip(exe, cases, d)
     assert np.all(np.sqrt(bx) <= np.sqrt(exact) + 2.0 ** -19 * L)  # conservative pruning
     unflagged = fl == 0
     err64 = np.abs(np.sqrt(d) - np.sqrt(ericson))
