@@ -196,6 +196,7 @@ def run_batch(args, samples):
     """64 displaced icospheres (4.1 M triangles), one grid each (84.8 M nodes), signed distance +
     triangle with FP64 fixup: the per-mesh reference-shaped call vs one batch."""
     import ssdf_amd as S
+    from ssdf_amd import meshgen as mg
     from ssdf_amd import workloads as W
     meshes, dims, boxes = W.batch64()
     nq = int(dims.prod(1).sum())
@@ -213,6 +214,22 @@ def run_batch(args, samples):
     res["per_mesh_calls_s"] = time.perf_counter() - t0
     res["per_mesh_calls_mqps"] = nq / res["per_mesh_calls_s"] / 1e6
     log(f"[bench] batch64 per-mesh calls: {res['per_mesh_calls_s']:.3f} s")
+    # the reference's own per-call GPU path (serial/bvh/bvh.cu:80-158, edf_bvh_cuda): host points in,
+    # plain FP32 unsigned distances min-accumulated into a host array, one call per mesh; here
+    # ssdf_edf_f over each mesh's grid nodes given as points (the points are the caller's data and
+    # are made before the clock starts)
+    pts = [mg.grid_points(int(d[0]), int(d[1]), int(d[2]), bx[:3], bx[3:]) for d, bx in zip(dims, boxes)]
+    eouts = [np.full(p[0].size, np.inf, np.float32) for p in pts]
+    for m, p, o in zip(meshes[:2], pts, eouts):
+        S.edf(*p, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, o)
+        o.fill(np.inf)
+    t0 = time.perf_counter()
+    for m, p, o in zip(meshes, pts, eouts):
+        S.edf(*p, m.s0, m.s1, m.s2, m.sx, m.sy, m.sz, o)
+    res["per_mesh_edf_calls_s"] = time.perf_counter() - t0
+    res["per_mesh_edf_calls_mqps"] = nq / res["per_mesh_edf_calls_s"] / 1e6
+    log(f"[bench] batch64 per-mesh edf calls (plain, host points): {res['per_mesh_edf_calls_s']:.3f} s")
+    del pts, eouts
     # one batch, host to host: upload + build all, one query launch, download
     off, total = S.Batch.grid_layout(dims)
     st = S.Stream()
@@ -253,7 +270,8 @@ def run_batch(args, samples):
     for (bk, hv) in per_mesh:
         key = "x".join(map(str, bk))
         bricks[key] = bricks.get(key, 0) + 1
-        budgets[str(hv)] = budgets.get(str(hv), 0) + 1
+        hk = str(hv) if hv >= 0 else "process-wide (8192)"
+        budgets[hk] = budgets.get(hk, 0) + 1
     res["schedule"] = {"blocks_per_cu": bpc, "units_per_grab": ch, "query_ms": round(tune_ms, 3),
                        "meshes_per_brick": bricks, "meshes_per_leaf_budget": budgets,
                        "resident_query_default_schedule_ms": round(default_ms, 3)}
@@ -276,6 +294,7 @@ def run_batch(args, samples):
                            "useful_box_frac": wc["packet"]["useful_box_frac"],
                            "useful_tri_frac": wc["packet"]["useful_tri_frac"]}
     res["speedup_resident_step_vs_per_mesh_calls"] = res["resident_step_mqps"] / res["per_mesh_calls_mqps"]
+    res["speedup_resident_step_vs_per_mesh_edf_calls"] = res["resident_step_mqps"] / res["per_mesh_edf_calls_mqps"]
     res["speedup_host_to_host_vs_per_mesh_calls"] = res["batch_host_to_host_mqps"] / res["per_mesh_calls_mqps"]
     ho, ht = out.download(), tri.download()
     for i, m in enumerate(meshes):

@@ -282,9 +282,11 @@ int ssdf_batch_get_mesh_schedule(ssdf_batch_t batch, int mesh, int *brick_x, int
 /* Autotuner of grid queries.  Per mesh, the brick (4x4x4, 8x4x2, 4x8x2, 2x4x8, 8x8x1), chosen from
  * one query per candidate in which every mesh's waves clock their own cycles (packet and per-query
  * kernels), so each mesh keeps its cheapest.  Then the leaf budget (4096, 8192, 16384) and the launch
- * (blocks per CU x units per grab), timed with events under the per-mesh bricks.  Keeps the result
- * (ssdf_batch_get_mesh_schedule reads it) and reports the launch schedule and its time.  Runs the
- * query 52 times. */
+ * (blocks per CU x units per grab), timed with events under the per-mesh bricks.  The schedule found
+ * is kept only if it beats the default one (4x4x4 bricks, process-wide budget, 8 blocks per CU, one
+ * unit per grab) over three alternating timed runs of each; otherwise the default is restored.  The
+ * kept schedule stays in the batch (ssdf_batch_get_mesh_schedule reads it); reports the launch
+ * schedule and its time.  Runs the query 64 times. */
 int ssdf_batch_autotune(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, const double *boxes,
                         const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
                         int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms);

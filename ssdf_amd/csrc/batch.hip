@@ -316,9 +316,10 @@ int stack_of(const ssdf_batch *b) {
 
 // Default: 8 blocks per CU (8 waves per SIMD, the packet kernel's occupancy), what the autotuner
 // picks on the 64-mesh batch in both modes.
+constexpr int kDefaultBlocksPerCu = 8;
 unsigned grid_blocks(const ssdf_batch *b, unsigned total_units, bool fix) {
     (void)fix;
-    const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu : 8;
+    const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu : kDefaultBlocksPerCu;
     const unsigned want = (unsigned)(b->num_cus * bpc);
     return std::max(1u, std::min(want, (total_units + 3) / 4));
 }
@@ -640,7 +641,36 @@ int ssdf_batch_autotune(ssdf_batch_t b, int mode, const ptrdiff_t *dims, const d
             if (ms < best) best = ms, bb = bpc, bc = ch;
         }
     }
-    b->blocks_per_cu = bb, b->chunk = bc;
+    // 4. each of the stages above picked on one timed run per candidate; the schedule is kept only
+    //    if it beats the default (4x4x4 bricks, the process-wide leaf budget, 8 blocks per CU, one
+    //    unit per grab) over three alternating runs each, so that run-to-run noise cannot leave the
+    //    batch slower than it started
+    const std::vector<int> tuned_brick = b->mesh_brick, tuned_heavy = b->mesh_heavy;
+    auto set_tuned = [&]() {
+        b->mesh_brick = tuned_brick, b->mesh_heavy = tuned_heavy;
+        b->blocks_per_cu = bb, b->chunk = bc;
+    };
+    auto set_default = [&]() {
+        std::fill(b->mesh_brick.begin(), b->mesh_brick.end(), brick_code(2, 2, 2));
+        std::fill(b->mesh_heavy.begin(), b->mesh_heavy.end(), -1);
+        b->blocks_per_cu = kDefaultBlocksPerCu, b->chunk = 1;
+    };
+    float t_tuned = 1e30f, t_default = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+        float ms = 0.0f;
+        set_default();
+        if ((rc = timed(ms))) return rc;
+        t_default = std::min(t_default, ms);
+        set_tuned();
+        if ((rc = timed(ms))) return rc;
+        t_tuned = std::min(t_tuned, ms);
+    }
+    if (t_default <= t_tuned) {
+        set_default();
+        bb = kDefaultBlocksPerCu, bc = 1, best = t_default;
+    } else {
+        best = t_tuned;
+    }
     if (best_blocks_per_cu) *best_blocks_per_cu = bb;
     if (best_units_per_grab) *best_units_per_grab = bc;
     if (best_ms) *best_ms = best;
