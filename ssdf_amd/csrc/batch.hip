@@ -24,6 +24,16 @@ using namespace ssdf_amd;
 constexpr size_t kConnStride = 64;
 static_assert(kConnStride >= 3 * kConnBytesPerCorner, "a triangle's corners fit its stride");
 
+// Work-unit order: the packet kernel takes its units through `order` and records each unit's
+// cycles in `unit_cost`; after it, the costliest units of each XCD range move to its front for the
+// next query with the same plan (`key`).  ctr: per range, counts of the 32 cost classes (zero
+// between queries), then per range two cursors and the lowest heavy class (4 words).
+struct UnitOrder {
+    unsigned *order = nullptr, *unit_cost = nullptr, *ctr = nullptr;
+    size_t cap = 0;
+    unsigned long long key = 0;
+};
+
 struct ssdf_batch {
     int nm = 0;
     bool with_normals = false;
@@ -61,6 +71,14 @@ struct ssdf_batch {
     std::vector<int> mesh_brick;       // per mesh: brick shape of its grid waves (brick_code; 0: 4x4x4)
     std::vector<int> mesh_heavy;       // per mesh: leaf budget of its packet waves (-1: process-wide)
     unsigned long long *cost = nullptr;  // per-mesh cycle counters of the autotuner
+    // Work-unit orders of eager queries and of the captured graph (each its own, so that neither
+    // sees an order made for the other's plan).
+    UnitOrder eager_order, graph_order;
+    // The captured graph's own descriptors and fixup queue: eager queries of another plan rewrite
+    // (and may reallocate) the shared ones between replays.
+    BatchMesh *graph_desc = nullptr;
+    unsigned *graph_fix = nullptr;
+    size_t graph_fix_cap = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
 };
@@ -78,7 +96,9 @@ void destroy_batch(ssdf_batch *b) {
     for (void *p : {(void *)b->s0, (void *)b->s1, (void *)b->s2, (void *)b->sx, (void *)b->sy, (void *)b->sz,
                     (void *)b->nodes, (void *)b->tris, (void *)b->metas, (void *)b->face_n, (void *)b->edge_n,
                     (void *)b->vertex_n, (void *)b->conn, (void *)b->ws, (void *)b->d_desc, (void *)b->queue, (void *)b->status,
-                    (void *)b->fix_list, (void *)b->cost})
+                    (void *)b->fix_list, (void *)b->cost, (void *)b->eager_order.order, (void *)b->eager_order.unit_cost,
+                    (void *)b->eager_order.ctr, (void *)b->graph_order.order, (void *)b->graph_order.unit_cost,
+                    (void *)b->graph_order.ctr, (void *)b->graph_desc, (void *)b->graph_fix})
         dev_free(p, st);
     for (int l = 1; l < ssdf_batch::kLanes; l++) dev_free(b->lane_ws[l], st);
     if (b->h_desc) (void)hipHostFree(b->h_desc);
@@ -308,6 +328,133 @@ int stage(ssdf_batch *b, const Plan &p, int qsrc, int mode, unsigned flags, cons
     return SSDF_OK;
 }
 
+// ---- cost-ordered work units ----
+// Range x of U units is [U x / 8, U (x + 1) / 8) (k_batch_packet's split); a unit's cost class is
+// the bit length of its cycle count (32 classes).  The units of a range's kHeavyClasses costliest
+// non-empty classes (within 2^kHeavyClasses of its costliest unit) move to the front of the range,
+// the others keep their places' order behind them: a heavy unit that starts late would run on alone
+// after every other wave has finished, while sorting all units by cost scatters the meshes over
+// time and loses the locality of the natural order (mesh after mesh, bricks in Morton order).
+constexpr int kOrderClasses = 32;
+constexpr int kHeavyClasses = 4;
+
+__device__ __forceinline__ unsigned unit_range(unsigned u, unsigned U) {
+    return (unsigned)((8ull * (u + 1ull) - 1ull) / U);
+}
+__device__ __forceinline__ unsigned cost_class(unsigned c) { return 31u - (unsigned)__clz(c | 1u); }
+
+__global__ __launch_bounds__(256) void k_order_identity(unsigned *order, unsigned U, unsigned *ctr) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < U) order[i] = i;
+    if (blockIdx.x == 0 && threadIdx.x < 8 * kOrderClasses) ctr[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_order_count(const unsigned *__restrict__ cost, unsigned U, unsigned *ctr) {
+    __shared__ unsigned h[8 * kOrderClasses];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const unsigned u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < U) atomicAdd(&h[unit_range(u, U) * kOrderClasses + cost_class(cost[u])], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&ctr[threadIdx.x], h[threadIdx.x]);
+}
+
+// One block of 8 threads, one per range: the lowest heavy class and the two cursors (heavy units
+// from the range's start, the others after them), then the counts cleared.
+__global__ __launch_bounds__(64) void k_order_offsets(unsigned U, unsigned *ctr) {
+    const unsigned x = threadIdx.x;
+    if (x >= 8) return;
+    unsigned *cnt = ctr + x * kOrderClasses;
+    int top = kOrderClasses - 1;
+    while (top > 0 && cnt[top] == 0u) top--;
+    const int thr = top - kHeavyClasses + 1;
+    unsigned heavy = 0;
+    for (int c = kOrderClasses - 1; c >= 0 && c >= thr; c--) heavy += cnt[c];
+    const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
+    unsigned *cur = ctr + 8 * kOrderClasses + 4 * x;
+    cur[0] = lo;          // heavy
+    cur[1] = lo + heavy;  // the rest
+    cur[2] = (unsigned)max(thr, 0);
+    for (int c = 0; c < kOrderClasses; c++) cnt[c] = 0u;
+}
+
+// Each wave's lanes that share a cursor take their places with one atomic (the lanes of a wave hold
+// consecutive units: at most two ranges and two buckets), in lane order.
+__global__ __launch_bounds__(256) void k_order_scatter(const unsigned *__restrict__ cost, unsigned U, unsigned *ctr,
+                                                       unsigned *order) {
+    const unsigned u = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lane = threadIdx.x & 63u;
+    unsigned key = 0xffffffffu;
+    if (u < U) {
+        const unsigned x = unit_range(u, U);
+        key = 4 * x + (cost_class(cost[u]) < ctr[8 * kOrderClasses + 4 * x + 2] ? 1u : 0u);
+    }
+    uint64_t todo = __ballot(u < U);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const unsigned k = (unsigned)__shfl((int)key, leader);
+        const uint64_t grp = __ballot(key == k);
+        unsigned base = 0;
+        if ((int)lane == leader) base = atomicAdd(&ctr[8 * kOrderClasses + k], (unsigned)__popcll(grp));
+        base = (unsigned)__shfl((int)base, leader);
+        if (key == k) order[base + (unsigned)__popcll(grp & ((1ull << lane) - 1ull))] = u;
+        todo &= ~grp;
+    }
+}
+
+// Key of a plan's work units: the order of one query serves the next only if their units agree.
+unsigned long long plan_key(const Plan &p, int qsrc) {
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ull; };
+    mix((unsigned long long)qsrc);
+    mix(p.total_units);
+    for (const BatchMesh &d : p.desc) {
+        mix((unsigned long long)d.mesh), mix(d.unit_start), mix((unsigned long long)d.brick);
+        mix((unsigned long long)d.nx << 32 | (unsigned)d.ny), mix((unsigned long long)d.nz);
+        mix((unsigned long long)d.npoints);
+    }
+    return h | 1ull;  // never 0 (no order yet)
+}
+
+// Order buffers for p's units; an identity order when the last order was made for another plan.
+int prepare_order(ssdf_batch *b, UnitOrder &o, const Plan &p, int qsrc, hipStream_t st, BatchArgs &a) {
+    const unsigned U = p.total_units;
+    if (o.cap < U) {
+        dev_free(o.order, b->stream), dev_free(o.unit_cost, b->stream);
+        o.order = o.unit_cost = nullptr, o.cap = 0, o.key = 0;
+        SSDF_HIP_TRY(dev_alloc((void **)&o.order, sizeof(unsigned) * U, b->stream));
+        SSDF_HIP_TRY(dev_alloc((void **)&o.unit_cost, sizeof(unsigned) * U, b->stream));
+        o.cap = U;
+    }
+    if (!o.ctr) SSDF_HIP_TRY(dev_alloc((void **)&o.ctr, sizeof(unsigned) * 2 * 8 * kOrderClasses, b->stream));
+    if (st != b->stream) {
+        hipEvent_t ev;
+        SSDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SSDF_HIP_TRY(hipEventRecord(ev, b->stream));
+        SSDF_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
+        (void)hipEventDestroy(ev);
+    }
+    const unsigned long long key = plan_key(p, qsrc);
+    if (key != o.key) {
+        hipLaunchKernelGGL(k_order_identity, dim3((U + 255) / 256), dim3(256), 0, st, o.order, U, o.ctr);
+        SSDF_HIP_TRY(hipGetLastError());
+        o.key = key;
+    }
+    a.order = o.order;
+    a.unit_cost = o.unit_cost;
+    return SSDF_OK;
+}
+
+// After a query: the next order from the cycles its units just recorded.
+int reorder_units(const UnitOrder &o, unsigned U, hipStream_t st) {
+    const unsigned g = (U + 255) / 256;
+    hipLaunchKernelGGL(k_order_count, dim3(g), dim3(256), 0, st, o.unit_cost, U, o.ctr);
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, st, U, o.ctr);
+    hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, st, o.unit_cost, U, o.ctr, o.order);
+    SSDF_HIP_TRY(hipGetLastError());
+    return SSDF_OK;
+}
+
 int stack_of(const ssdf_batch *b) {
     if (b->maxdepth <= 64) return 64;
     if (b->maxdepth <= 128) return 128;
@@ -347,7 +494,9 @@ int run_query(ssdf_batch *b, int qsrc, int mode, const ptrdiff_t *dims, const do
     }
     const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
     a.cost = cost;
-    return launch_batch(a, b->d_desc, qsrc, mode, fix, stack, grid_blocks(b, p.total_units, fix), st);
+    if (int rc = prepare_order(b, b->eager_order, p, qsrc, st, a)) return rc;
+    if (int rc = launch_batch(a, b->d_desc, qsrc, mode, fix, stack, grid_blocks(b, p.total_units, fix), st)) return rc;
+    return reorder_units(b->eager_order, p.total_units, st);
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -698,20 +847,41 @@ int ssdf_batch_capture(ssdf_batch_t b, int rebuild, int mode, const ptrdiff_t *d
     if (int rc = stage(b, p, QSRC_GRID, mode, flags, nullptr, nullptr, nullptr, out_d, out_tri, nullptr, nullptr,
                        nullptr, st, a))
         return rc;
-    SSDF_HIP_TRY(hipStreamSynchronize(st));  // descriptors and buffers in place before capture
     const int stack = stack_of(b);
     if (!stack) {
         set_error("batch: LBVH deeper than 128 levels");
         return SSDF_ERR_DEPTH;
     }
+    // the previous graph (and any replay of it still running) goes before its buffers are reused
+    SSDF_HIP_TRY(hipDeviceSynchronize());
     if (b->graph) (void)hipGraphExecDestroy(b->graph), b->graph = nullptr;
     if (b->graph_def) (void)hipGraphDestroy(b->graph_def), b->graph_def = nullptr;
+    // the graph's own descriptors (a copy of the ones stage just uploaded) and fixup queue
+    if (!b->graph_desc) SSDF_HIP_TRY(dev_alloc((void **)&b->graph_desc, sizeof(BatchMesh) * b->nm, b->stream));
+    if (b->graph_fix_cap < b->fix_cap) {
+        dev_free(b->graph_fix, b->stream);
+        b->graph_fix = nullptr, b->graph_fix_cap = 0;
+        SSDF_HIP_TRY(dev_alloc((void **)&b->graph_fix, sizeof(unsigned) * 4 * b->fix_cap, b->stream));
+        b->graph_fix_cap = b->fix_cap;
+    }
+    SSDF_HIP_TRY(hipStreamSynchronize(b->stream));
+    SSDF_HIP_TRY(hipMemcpyAsync(b->graph_desc, b->d_desc, sizeof(BatchMesh) * p.desc.size(), hipMemcpyDeviceToDevice,
+                                st));
+    a.base.fix_list = b->graph_fix;
+    a.base.fix_cap = (unsigned)std::min<size_t>(b->graph_fix_cap, 0xffffffffu);
+    a.base.heavy_cap = a.base.fix_cap / 2;
+    // the graph's own unit order, starting from the identity (replays reorder it for the next one)
+    b->graph_order.key = 0;
+    if (p.total_units > 0)
+        if (int rc = prepare_order(b, b->graph_order, p, QSRC_GRID, st, a)) return rc;
+    SSDF_HIP_TRY(hipStreamSynchronize(st));  // descriptors and buffers in place before capture
     const bool fix = (flags & SSDF_FLAG_FIXUP) != 0;
     SSDF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = SSDF_OK;
     if (rebuild) rc = build_lanes(b, b->with_normals, st);  // workspaces sized since creation
-    if (!rc && p.total_units > 0) rc = launch_batch(a, b->d_desc, QSRC_GRID, mode, fix, stack,
+    if (!rc && p.total_units > 0) rc = launch_batch(a, b->graph_desc, QSRC_GRID, mode, fix, stack,
                                                     grid_blocks(b, p.total_units, fix), st);
+    if (!rc && p.total_units > 0) rc = reorder_units(b->graph_order, p.total_units, st);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &g);
     if (rc) {

@@ -977,7 +977,9 @@ __device__ __forceinline__ void batch_args(const BatchArgs &b, const BatchMesh &
 // Persistent: every wave takes `chunk` units at a time from a work queue until all are done.  The
 // units are split into 8 contiguous ranges with a counter each; block b starts on range b % 8 (the
 // XCD the hardware dispatches it to, so each XCD's L2 holds mostly its range's meshes) and moves
-// on to the next ranges when its own is exhausted.
+// on to the next ranges when its own is exhausted.  With an order, position p of a range is work
+// unit order[p] (a permutation within the range: the batch puts the units its previous query
+// measured costliest first), and each wave records its unit's cycles in unit_cost.
 template <int STACK, int QSRC, int OUT, bool FIX>
 // Occupancy, measured on the 64-mesh batch (FP32 + fixup, resident query): 8 waves per SIMD (64
 // VGPRs, the rest spilled outside the traversal loop) 76.5 ms, 7 waves 79.1 ms, 6 waves 83.2 ms,
@@ -999,7 +1001,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             const unsigned u0 = lo + grab;
             if (grab >= hi - lo) break;
             const unsigned u1 = min(u0 + b.chunk, hi);
-            for (unsigned u = u0; u < u1; u++) {
+            for (unsigned pos = u0; pos < u1; pos++) {
+                const unsigned u = b.order ? b.order[pos] : pos;
                 int m0 = 0, m1 = b.nmeshes;
                 while (m1 - m0 > 1) {
                     const int mid = (m0 + m1) >> 1;
@@ -1011,10 +1014,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 const BatchMesh &bm = meshes[m0];
                 QueryArgs a;
                 batch_args(b, bm, a);
-                const unsigned long long t0 = b.cost ? clock64() : 0ull;
+                const unsigned long long t0 = (b.cost || b.unit_cost) ? clock64() : 0ull;
                 packet_wave<STACK, QSRC, OUT, FIX>(nodes + bm.node_off, tris + bm.tri_off, a, u - bm.unit_start,
                                                    cand);
-                if (b.cost && lane == 0) atomicAdd(b.cost + bm.mesh, (unsigned long long)(clock64() - t0));
+                if (lane == 0 && (b.cost || b.unit_cost)) {
+                    const unsigned long long dt = clock64() - t0;
+                    if (b.cost) atomicAdd(b.cost + bm.mesh, dt);
+                    if (b.unit_cost) b.unit_cost[u] = (unsigned)min(dt, 0xffffffffull);
+                }
             }
         }
     }

@@ -164,6 +164,8 @@ struct BatchArgs {
     unsigned chunk;   // units per grab
     int nlev;         // LDS levels of the per-query kernel: the deepest tree of the batch
     unsigned long long *cost;  // optional, per mesh: shader-clock cycles its waves spent (autotuning)
+    const unsigned *order;     // optional: position -> work unit, each XCD range's units inside it
+    unsigned *unit_cost;       // optional, per unit: shader-clock cycles of its packet wave (saturated)
 };
 
 // Launches the reset, the persistent packet kernel and the queue kernel of a batch.

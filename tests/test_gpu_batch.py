@@ -147,6 +147,74 @@ def test_batch_graph_replays_match_eager(meshes):
         st.synchronize()
 
 
+@pytest.mark.parametrize("flags", [S.FLAG_FIXUP, 0])
+def test_batch_cost_ordered_units(meshes, flags):
+    """Each query takes its work units in the order the previous query with the same plan measured
+    (costliest first within each XCD range), eager and graph alike: every later query, interleaved
+    with queries of another plan and with graph replays, writes every output (sentinel-filled
+    before each call) bit for bit as the first one, in grid and point queries."""
+    dims, boxes = _grids(meshes)
+    off, total = S.Batch.grid_layout(dims)
+    dims2 = dims + 3  # another plan: other units
+    off2, total2 = S.Batch.grid_layout(dims2)
+    b = S.Batch(meshes)
+    st = S.Stream()
+    out, tri = S.DeviceArray(total, np.float32), S.DeviceArray(total, np.int32)
+    out2, tri2 = S.DeviceArray(total2, np.float32), S.DeviceArray(total2, np.int32)
+
+    def query(d, bx, o, t, n, tot):
+        o.upload(np.full(tot, np.nan, np.float32))
+        t.upload(np.full(tot, -7, np.int32))
+        b.query_grids(S.MODE_SDF, d, bx, n, o, t, flags=flags, stream=st)
+        st.synchronize()
+        assert b.last_stats()[1] == 0
+        return o.download(), t.download()
+
+    want = query(dims, boxes, out, tri, off, total)
+    want2 = query(dims2, boxes, out2, tri2, off2, total2)
+    for rep in range(3):
+        got = query(dims, boxes, out, tri, off, total)
+        assert np.array_equal(got[0], want[0], equal_nan=True) and np.array_equal(got[1], want[1]), rep
+        if rep == 1:
+            got2 = query(dims2, boxes, out2, tri2, off2, total2)
+            assert np.array_equal(got2[0], want2[0], equal_nan=True) and np.array_equal(got2[1], want2[1])
+    # the empty mesh's slice is the only one left at the sentinel
+    e = 4
+    assert np.all(np.isnan(want[0][off[e]:off[e] + int(dims[e].prod())]))
+    assert np.count_nonzero(np.isnan(want[0])) == int(dims[e].prod())
+    # graph replays keep an order of their own, eager queries of another plan between them
+    go, gt = S.DeviceArray(total, np.float32), S.DeviceArray(total, np.int32)
+    b.capture(False, S.MODE_SDF, dims, boxes, off, go, gt, flags=flags, stream=st)
+    for rep in range(4):
+        go.upload(np.full(total, np.nan, np.float32))
+        gt.upload(np.full(total, -7, np.int32))
+        b.replay(st)
+        st.synchronize()
+        assert np.array_equal(go.download(), want[0], equal_nan=True) and np.array_equal(gt.download(), want[1]), rep
+        got2 = query(dims2, boxes, out2, tri2, off2, total2)
+        assert np.array_equal(got2[0], want2[0], equal_nan=True)
+    # points: the same across calls
+    rng = np.random.default_rng(9)
+    counts = [3000, 10000, 650, 5000, 0, 1290]  # none for the empty mesh (4)
+    po = np.zeros(len(meshes) + 1, np.int64)
+    po[1:] = np.cumsum(counts)
+    Np = int(po[-1])
+    P = rng.uniform(-1.2, 1.2, (Np, 3)).astype(np.float32)
+    x, y, z = (S.DeviceArray.from_host(np.ascontiguousarray(P[:, d])) for d in range(3))
+    od, ot = S.DeviceArray(Np, np.float32), S.DeviceArray(Np, np.int32)
+    first = None
+    for rep in range(3):
+        od.upload(np.full(Np, np.nan, np.float32))
+        ot.upload(np.full(Np, -7, np.int32))
+        b.query_points(S.MODE_SDF, po, x, y, z, od, ot, flags=flags, stream=st)
+        st.synchronize()
+        got = od.download(), ot.download()
+        if first is None:
+            first = got
+            assert not np.any(np.isnan(got[0])) and np.all(got[1] >= 0)
+        assert np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]), rep
+
+
 def test_batch_autotune_and_schedule(meshes):
     dims, boxes = _grids(meshes)
     off, total = S.Batch.grid_layout(dims)
