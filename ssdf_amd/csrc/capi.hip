@@ -21,7 +21,7 @@ namespace {
 
 // Query workspace a handle keeps between calls (grown on demand, stream-ordered).
 struct Workspace {
-    unsigned *status = nullptr;  // [0] queue count, [1] kernel error flag, [2] queue read position
+    unsigned *status = nullptr;  // kStatusBytes: status words and the wave-queue counters (query.hpp)
     unsigned *list = nullptr;
     size_t cap = 0;  // words of list (4 per queue entry)
     void *sort = nullptr;  // Morton sort of scattered query points
@@ -30,10 +30,11 @@ struct Workspace {
 
 __global__ void k_reset_status(unsigned *status) {
     if (threadIdx.x < 4) status[threadIdx.x] = 0u;
+    if (threadIdx.x >= 4 && threadIdx.x < 12) status[kWaveQueueWord * (threadIdx.x - 3)] = 0u;
 }
 
 int ws_reserve(Workspace &w, size_t n, hipStream_t stream) {
-    if (!w.status) SSDF_HIP_TRY(dev_alloc((void **)&w.status, 64, stream));
+    if (!w.status) SSDF_HIP_TRY(dev_alloc((void **)&w.status, kStatusBytes, stream));
     if (w.cap < n) {
         dev_free(w.list, stream);
         w.list = nullptr;
@@ -181,6 +182,7 @@ void fill_args(const DevMesh &m, QueryArgs &a, const QueryWs &w, hipStream_t str
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(64), 0, stream, w.status);
     a.fix_count = w.status;
     a.err_flag = (int *)(w.status + 1);
+    a.wqueue = w.status + kWaveQueueWord;
     a.fix_list = w.list;
     a.fix_cap = (unsigned)std::min<size_t>(w.entries, 0xffffffffu);
     a.heavy_iters = heavy_leaves();

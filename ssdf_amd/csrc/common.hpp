@@ -334,13 +334,18 @@ __host__ __device__ __forceinline__ float ordered_to_float(int i) {
     return u.f;
 }
 
-// Blocks of one launch are handed to the 8 XCDs round-robin (block b -> XCD b mod 8).  This
-// bijection gives each XCD a contiguous range of logical tiles so its private L2 sees one
-// spatially compact region of the mesh.
+// Blocks of one launch are handed to the 8 XCDs round-robin (block b -> XCD b mod 8), each XCD
+// taking its blocks in order.  This bijection gives XCD x the logical chunks x, x + 8, x + 16, ...
+// of kXcdChunk consecutive tiles, so its private L2 sees one compact region at a time while the 8
+// XCDs sweep the grid side by side (one contiguous eighth per XCD left the XCDs with cheap
+// eighths idle for the last third of the launch).  Blocks past the last whole stripe of 8 chunks
+// keep their own index.
+constexpr unsigned kXcdChunk = 128;
 __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
-    const unsigned q = nblocks >> 3, r = nblocks & 7u;
+    const unsigned span = 8u * kXcdChunk;
+    if (b >= nblocks / span * span) return b;
     const unsigned x = b & 7u, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
+    return ((k / kXcdChunk) * 8u + x) * kXcdChunk + k % kXcdChunk;
 }
 
 }  // namespace ssdf_amd

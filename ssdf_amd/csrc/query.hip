@@ -22,6 +22,7 @@
 #include "radix.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 
 namespace ssdf_amd {
@@ -512,6 +513,25 @@ template <int STACK, int QSRC, int OUT, bool FIX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
+    if (a.wqueue) {
+        // persistent: one wave at a time from the XCD's range of the wave order, then the others'
+        const int lane = threadIdx.x & 63;
+        const unsigned U = a.total_waves;
+        const unsigned x0 = blockIdx.x & 7u;
+        for (unsigned k = 0; k < 8; k++) {
+            const unsigned x = (x0 + k) & 7u;
+            const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
+            const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
+            for (;;) {
+                unsigned grab = 0;
+                if (lane == 0) grab = atomicAdd(&a.wqueue[32 * x], 1u);
+                grab = (unsigned)uniform_i(__shfl((int)grab, 0));
+                if (grab >= hi - lo) break;
+                packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, lo + grab, cand);
+            }
+        }
+        return;
+    }
     const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
     packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, blk * kWaves + (threadIdx.x >> 6), cand);
 }
@@ -1152,6 +1172,23 @@ int sort_points(const MeshMeta *meta, long long n, const float *x, const float *
 // ------------------------------------------------------------------------------------------------
 // Host dispatch
 // ------------------------------------------------------------------------------------------------
+// Compute units of the current device (cached per device).
+static unsigned device_cus() {
+    static std::atomic<unsigned> cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256u;
+    unsigned c = cus[dev].load(std::memory_order_relaxed);
+    if (!c) {
+        int n = 0;
+        c = (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) ? (unsigned)n
+                                                                                                          : 256u;
+        cus[dev].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
+// Resident blocks per compute unit of k_query_packet: 8 waves per SIMD, 4 waves a block.
+constexpr unsigned kBlocksPerCu = 8;
+
 template <int STACK, int QSRC, int OUT, bool FIX>
 static void launch_packet(QueryArgs &a, unsigned nblocks, hipStream_t s) {
     a.nblocks = nblocks;
@@ -1237,7 +1274,17 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
         }
         nwaves = (unsigned)w;
     }
-    const unsigned nblocks = (nwaves + kWaves - 1) / kWaves;
+    unsigned nblocks = (nwaves + kWaves - 1) / kWaves;
+    // More waves than the GPU holds at once (8 per SIMD): a persistent launch of one resident set,
+    // the waves handed out per XCD with stealing, instead of one block per 4 waves dispatched in
+    // order (which leaves XCDs whose share of the grid is cheaper idle).
+    const unsigned resident = device_cus() * kBlocksPerCu;
+    a.total_waves = nwaves;
+    if (nblocks > resident && a.wqueue) {
+        nblocks = resident;
+    } else {
+        a.wqueue = nullptr;
+    }
     int rc;
     if (depth <= 64) {
         rc = qsrc == QSRC_POINTS ? dispatch_packet_out<64, QSRC_POINTS>(a, out, fix, nblocks, stream)

@@ -62,6 +62,11 @@ struct QueryArgs {
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
     int heavy_iters;      // leaves a packet wave walks before handing its queries over (0: never)
     unsigned nblocks;  // launched blocks (XCD remap)
+    // Persistent launch (wqueue != nullptr): the waves of total_waves go out in 8 contiguous ranges,
+    // one per XCD, through the counters wqueue[32 x] (128 bytes apart); a wave whose range is
+    // exhausted moves on to the next ones.
+    unsigned *wqueue;
+    unsigned total_waves;
     unsigned flags;    // QF_* of the call
     // winding numbers (gwn.hip)
     const GwnNode *gwn;
@@ -197,6 +202,12 @@ struct QueryWs {
 
 constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Status area of a query workspace: words 0-3 (queue count, kernel error flag, queue read position),
+// then the 8 wave-queue counters of a persistent launch at words 32 (x + 1), each on a 128-byte line
+// of its own (cleared with the status words before every query).
+constexpr size_t kStatusBytes = 1280;
+constexpr unsigned kWaveQueueWord = 32;
+
 // Queue entries reserved for nq queries (FIX near-ties that overflow the candidate list, and the
 // queries of packet waves past their leaf budget): an eighth of the queries, at least 64 Ki.  Past
 // that a near-tie keeps its FP32 result and sets bit 4 of the kernel error flag, and a heavy query
@@ -205,7 +216,7 @@ constexpr size_t fixup_entries(size_t nq) { return std::min(nq, std::max<size_t>
 
 // Bytes of a contiguous QueryWs for nq queries (sorted: with room for the point sort).
 constexpr size_t query_ws_bytes(size_t nq, bool sorted) {
-    return 256 + round256(16 * std::max<size_t>(fixup_entries(nq), 1)) +
+    return kStatusBytes + round256(16 * std::max<size_t>(fixup_entries(nq), 1)) +
            (sorted ? round256(sort_ws_bytes((long long)nq)) : 0);
 }
 
@@ -214,10 +225,10 @@ inline QueryWs carve_ws(void *base, size_t nq, bool sorted) {
     QueryWs w;
     char *p = (char *)base;
     w.status = (unsigned *)p;
-    w.list = (unsigned *)(p + 256);
+    w.list = (unsigned *)(p + kStatusBytes);
     w.entries = std::max<size_t>(fixup_entries(nq), 1);
     if (sorted) {
-        w.sort = p + 256 + round256(16 * w.entries);
+        w.sort = p + kStatusBytes + round256(16 * w.entries);
         w.sort_bytes = round256(sort_ws_bytes((long long)nq));
     }
     return w;
