@@ -203,11 +203,9 @@ __device__ __forceinline__ float solid_angle(float qx, float qy, float qz, const
 }
 
 template <int STACK, int QSRC, typename QT, int MODE>
-__global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, const TriP *__restrict__ tris,
-                                             QueryArgs a) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
-    const unsigned wave_id = blk * kGwnWaves + (unsigned)wv;
+__device__ __forceinline__ void gwn_wave(const GwnNode *__restrict__ g, const TriP *__restrict__ tris,
+                                         const QueryArgs &a, const unsigned wave_id) {
+    const int lane = threadIdx.x & 63;
     long long qid = 0;
     QT qx0 = QT(0), qy0 = QT(0), qz0 = QT(0);
     const bool valid = lane_query<QSRC, QT>(a, wave_id, lane, qid, qx0, qy0, qz0);
@@ -275,15 +273,70 @@ __global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, cons
     }
 }
 
+// One block per kGwnWaves waves, or (a.persistent) one resident set taking waves from 8 per-XCD
+// ranges with stealing, as k_query_packet does.
+template <int STACK, int QSRC, typename QT, int MODE>
+__global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, const TriP *__restrict__ tris,
+                                             QueryArgs a) {
+    if (a.persistent) {
+        const int lane = threadIdx.x & 63;
+        const unsigned U = a.total_waves, x0 = blockIdx.x & 7u;
+        for (unsigned k = 0; k < 8; k++) {
+            const unsigned x = (x0 + k) & 7u;
+            const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
+            const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
+            for (;;) {
+                unsigned grab = 0;
+                if (lane == 0) grab = atomicAdd(&a.wqueue[32 * x], 1u);
+                grab = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)grab, 0));
+                if (grab >= hi - lo) break;
+                gwn_wave<STACK, QSRC, QT, MODE>(g, tris, a, lo + grab);
+            }
+        }
+        return;
+    }
+    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
+    gwn_wave<STACK, QSRC, QT, MODE>(g, tris, a, blk * kGwnWaves + (unsigned)(threadIdx.x >> 6));
+}
+
+__global__ void k_reset_wqueue(unsigned *wq) {
+    if (threadIdx.x < 8) wq[32 * threadIdx.x] = 0u;
+}
+
+// Blocks of kernel k resident on one compute unit (cached per instance).
+template <int STACK, int QSRC, typename QT, int MODE>
+unsigned gwn_blocks_per_cu() {
+    static const unsigned n = [] {
+        int b = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b, reinterpret_cast<const void *>(&k_gwn<STACK, QSRC, QT, MODE>), 64 * kGwnWaves, 0);
+        return e == hipSuccess && b > 0 ? (unsigned)b : 1u;
+    }();
+    return n;
+}
+
+template <int STACK, int QSRC, typename QT, int MODE>
+void launch_mode(const DevMesh &m, QueryArgs a, unsigned nblocks, hipStream_t stream) {
+    // more waves than the GPU holds at once: a persistent launch (the wave-queue counters of the
+    // query's workspace, which a distance query before may have used, cleared first)
+    const unsigned resident = device_cus() * gwn_blocks_per_cu<STACK, QSRC, QT, MODE>();
+    a.persistent = nblocks > resident && a.wqueue ? 1u : 0u;
+    if (a.persistent) {
+        hipLaunchKernelGGL(k_reset_wqueue, dim3(1), dim3(64), 0, stream, a.wqueue);
+        nblocks = resident;
+    }
+    hipLaunchKernelGGL((k_gwn<STACK, QSRC, QT, MODE>), dim3(nblocks), dim3(64 * kGwnWaves), 0, stream, m.gwn, m.tris,
+                       a);
+}
+
 template <int STACK, int QSRC, typename QT>
 void dispatch_mode(const DevMesh &m, const QueryArgs &a, int mode, unsigned nblocks, hipStream_t stream) {
-    const dim3 grid(nblocks), block(64 * kGwnWaves);
     if (mode == GW_WINDING)
-        hipLaunchKernelGGL((k_gwn<STACK, QSRC, QT, GW_WINDING>), grid, block, 0, stream, m.gwn, m.tris, a);
+        launch_mode<STACK, QSRC, QT, GW_WINDING>(m, a, nblocks, stream);
     else if (mode == GW_INSIDE)
-        hipLaunchKernelGGL((k_gwn<STACK, QSRC, QT, GW_INSIDE>), grid, block, 0, stream, m.gwn, m.tris, a);
+        launch_mode<STACK, QSRC, QT, GW_INSIDE>(m, a, nblocks, stream);
     else
-        hipLaunchKernelGGL((k_gwn<STACK, QSRC, QT, GW_SIGN>), grid, block, 0, stream, m.gwn, m.tris, a);
+        launch_mode<STACK, QSRC, QT, GW_SIGN>(m, a, nblocks, stream);
 }
 
 template <int QSRC, typename QT>
@@ -364,6 +417,7 @@ int launch_gwn(DevMesh &m, QueryArgs &a, int qsrc, int mode, int q_is_double, hi
     }
     const unsigned nblocks = (nwaves + kGwnWaves - 1) / kGwnWaves;
     a.nblocks = nblocks;
+    a.total_waves = nwaves;
     int rc;
     if (m.dtype == 0)
         rc = qsrc == QSRC_POINTS ? dispatch_stack<QSRC_POINTS, float>(m, a, mode, nblocks, stream)

@@ -513,7 +513,7 @@ template <int STACK, int QSRC, int OUT, bool FIX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
-    if (a.wqueue) {
+    if (a.persistent) {
         // persistent: one wave at a time from the XCD's range of the wave order, then the others'
         const int lane = threadIdx.x & 63;
         const unsigned U = a.total_waves;
@@ -1173,7 +1173,7 @@ int sort_points(const MeshMeta *meta, long long n, const float *x, const float *
 // Host dispatch
 // ------------------------------------------------------------------------------------------------
 // Compute units of the current device (cached per device).
-static unsigned device_cus() {
+unsigned device_cus() {
     static std::atomic<unsigned> cus[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256u;
@@ -1280,11 +1280,8 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
     // order (which leaves XCDs whose share of the grid is cheaper idle).
     const unsigned resident = device_cus() * kBlocksPerCu;
     a.total_waves = nwaves;
-    if (nblocks > resident && a.wqueue) {
-        nblocks = resident;
-    } else {
-        a.wqueue = nullptr;
-    }
+    a.persistent = nblocks > resident && a.wqueue ? 1u : 0u;
+    if (a.persistent) nblocks = resident;
     int rc;
     if (depth <= 64) {
         rc = qsrc == QSRC_POINTS ? dispatch_packet_out<64, QSRC_POINTS>(a, out, fix, nblocks, stream)

@@ -62,11 +62,12 @@ struct QueryArgs {
     long long max_iters;  // traversal steps bound (2 x nodes + slack): a malformed tree cannot hang
     int heavy_iters;      // leaves a packet wave walks before handing its queries over (0: never)
     unsigned nblocks;  // launched blocks (XCD remap)
-    // Persistent launch (wqueue != nullptr): the waves of total_waves go out in 8 contiguous ranges,
-    // one per XCD, through the counters wqueue[32 x] (128 bytes apart); a wave whose range is
-    // exhausted moves on to the next ones.
+    // Persistent launch (persistent != 0): the waves of total_waves go out in 8 contiguous ranges,
+    // one per XCD, through the counters wqueue[32 x] (128 bytes apart, zero at the launch); a wave
+    // whose range is exhausted moves on to the next ones.
     unsigned *wqueue;
     unsigned total_waves;
+    unsigned persistent;
     unsigned flags;    // QF_* of the call
     // winding numbers (gwn.hip)
     const GwnNode *gwn;
@@ -74,6 +75,9 @@ struct QueryArgs {
     float *out_w;         // GW_WINDING
     unsigned char *out_in;  // GW_INSIDE
 };
+
+// Compute units of the current device (cached per device).
+unsigned device_cus();
 
 int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned flags, int out_is_double,
                  int q_is_double, hipStream_t stream);
