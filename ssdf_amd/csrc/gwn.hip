@@ -31,7 +31,7 @@ constexpr int kScanItems = 4;
 constexpr int kGwnScanTile = kGwnScanBlock * kScanItems;
 constexpr int kCh = 8;  // area vector (3), area, area * centroid (3), unused
 constexpr float kInv4Pi = 0.07957747154594766788f;
-constexpr int kGwnWaves = 4;
+constexpr int kGwnWaves = kWavesPerBlock;
 // far-field acceptance |c - q| > beta r; order-1 terms at beta = 3 keep |w error| well below the
 // 1/2 threshold away from the surface (tests/test_gpu_gwn.py bounds it against the exact sum)
 constexpr float kGwnBeta = 3.0f;
@@ -278,25 +278,8 @@ __device__ __forceinline__ void gwn_wave(const GwnNode *__restrict__ g, const Tr
 template <int STACK, int QSRC, typename QT, int MODE>
 __global__ __launch_bounds__(256) void k_gwn(const GwnNode *__restrict__ g, const TriP *__restrict__ tris,
                                              QueryArgs a) {
-    if (a.persistent) {
-        const int lane = threadIdx.x & 63;
-        const unsigned U = a.total_waves, x0 = blockIdx.x & 7u;
-        for (unsigned k = 0; k < 8; k++) {
-            const unsigned x = (x0 + k) & 7u;
-            const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
-            const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
-            for (;;) {
-                unsigned grab = 0;
-                if (lane == 0) grab = atomicAdd(&a.wqueue[32 * x], 1u);
-                grab = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)grab, 0));
-                if (grab >= hi - lo) break;
-                gwn_wave<STACK, QSRC, QT, MODE>(g, tris, a, lo + grab);
-            }
-        }
-        return;
-    }
-    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
-    gwn_wave<STACK, QSRC, QT, MODE>(g, tris, a, blk * kGwnWaves + (unsigned)(threadIdx.x >> 6));
+    WaveSource src(a);
+    for (unsigned w; src.next(a, w);) gwn_wave<STACK, QSRC, QT, MODE>(g, tris, a, w);
 }
 
 __global__ void k_reset_wqueue(unsigned *wq) {

@@ -27,7 +27,7 @@
 
 namespace ssdf_amd {
 
-constexpr int kWaves = 4;
+constexpr int kWaves = kWavesPerBlock;
 // FIX candidate list: 20 KB of LDS per block (5 KB per wave), the most that lets LDS allow 8 waves
 // per SIMD (the packet kernels ask the compiler for 8: 64 VGPRs).  A query with more than kCand
 // triangles in its band goes to the per-query kernel, which re-walks the tree in FP64.  Measured at
@@ -513,27 +513,9 @@ template <int STACK, int QSRC, int OUT, bool FIX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
-    if (a.persistent) {
-        // persistent: one wave at a time from the XCD's range of the wave order, then the others'
-        const int lane = threadIdx.x & 63;
-        const unsigned U = a.total_waves;
-        const unsigned x0 = blockIdx.x & 7u;
-        for (unsigned k = 0; k < 8; k++) {
-            const unsigned x = (x0 + k) & 7u;
-            const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
-            const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
-            for (;;) {
-                unsigned grab = 0;
-                if (lane == 0) grab = atomicAdd(&a.wqueue[32 * x], 1u);
-                grab = (unsigned)uniform_i(__shfl((int)grab, 0));
-                if (grab >= hi - lo) break;
-                packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, lo + grab, cand);
-            }
-        }
-        return;
-    }
-    const unsigned blk = xcd_remap(blockIdx.x, a.nblocks);
-    packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, blk * kWaves + (threadIdx.x >> 6), cand);
+    // one call site of the walk for both launch kinds (inlined twice, it spilled three times as much)
+    WaveSource src(a);
+    for (unsigned w; src.next(a, w);) packet_wave<STACK, QSRC, OUT, FIX>(nodes, tris, a, w, cand);
 }
 
 // ------------------------------------------------------------------------------------------------

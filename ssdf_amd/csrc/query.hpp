@@ -177,6 +177,40 @@ struct BatchArgs {
     unsigned *unit_cost;       // optional, per unit: shader-clock cycles of its packet wave (saturated)
 };
 
+// Waves per block of the packet and winding-number kernels.
+constexpr unsigned kWavesPerBlock = 4;
+
+// The waves a block of a packet or winding-number launch answers: its own 4 (one block per 4
+// waves, block ids remapped over the XCDs), or, in a persistent launch, one at a time from the
+// XCD's range of the wave order and then from the others' (QueryArgs::wqueue).
+struct WaveSource {
+    unsigned k = 0;      // persistent: ranges exhausted so far
+    bool done = false;   // one block per kWavesPerBlock waves: its wave handed out
+    __device__ explicit WaveSource(const QueryArgs &) {}
+    __device__ bool next(const QueryArgs &a, unsigned &w) {
+        if (!a.persistent) {
+            if (done) return false;
+            done = true;
+            w = xcd_remap(blockIdx.x, a.nblocks) * kWavesPerBlock + (threadIdx.x >> 6);
+            return true;
+        }
+        const unsigned U = a.total_waves, x0 = blockIdx.x & 7u;
+        for (; k < 8; k++) {
+            const unsigned x = (x0 + k) & 7u;
+            const unsigned lo = (unsigned)(((unsigned long long)U * x) >> 3);
+            const unsigned hi = (unsigned)(((unsigned long long)U * (x + 1)) >> 3);
+            unsigned grab = 0;
+            if ((threadIdx.x & 63) == 0) grab = atomicAdd(&a.wqueue[32 * x], 1u);
+            grab = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)grab, 0));
+            if (grab < hi - lo) {
+                w = lo + grab;
+                return true;
+            }
+        }
+        return false;
+    }
+};
+
 // Launches the reset, the persistent packet kernel and the queue kernel of a batch.
 // meshes: device descriptor array of b.nmeshes entries; stack: 64 or 128.
 int launch_batch(const BatchArgs &b, const BatchMesh *meshes, int qsrc, int out, bool fix, int stack,
