@@ -292,10 +292,6 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     float bestd = SSDF_INF;  // FIX: FP32 best distance found
     // Plane bound of the lane's pruning bound (plane_bound): a triangle whose plane lies farther than
     // this from every lane's query is skipped by the whole wave after 7 of its ~65 instructions.
-    // Grid waves only: in the point kernels with fixup it raises the scalar register pressure of
-    // the leaf loop into spills (8 lane reads per triangle), which cost more than the skips save
-    // (measured on 10^8 uniform points around the 200 beams: 105 -> 109 ms).
-    constexpr bool kPlaneSkip = QSRC == QSRC_GRID;
     float rb = valid ? plane_bound(__builtin_amdgcn_sqrtf(init2), babs) : -SSDF_INF;
 
 
@@ -349,10 +345,8 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
             const int k = k0 + j;
             const TriP tr = tris[k];
             const TriPlane pl = tri_plane(qx, qy, qz, tr);
-            if (kPlaneSkip) {
-                tri_loads_issued(tr);
-                if (__ballot(fabsf(pl.h) <= rb) == 0) continue;
-            }
+            tri_loads_issued(tr);
+            if (__ballot(fabsf(pl.h) <= rb) == 0) continue;
             const float d2 = tri_d2_rest(qx, qy, qz, tr, pl);
             if (!FIX) {
                 if (d2 < best) best = d2, bk = k, rb = plane_bound(__builtin_amdgcn_sqrtf(d2), babs);
