@@ -582,6 +582,53 @@ def test_mesh_brick_schedules_bit_identical(brick):
     assert np.all(res[1][1] >= 0)  # every node written
 
 
+@pytest.mark.parametrize("mode", [S.MODE_SDF, S.MODE_EDF])
+def test_persistent_launch_equals_small_launches(mode):
+    """A point query of more waves than the GPU holds at once (over 2048 blocks of 4 waves, i.e.
+    more than 524,288 points) runs as one resident set that takes its waves from per-XCD queues.
+    FP32 + fixup results are canonical, so they equal, bit for bit, the same points answered in
+    chunks small enough for one block per 4 waves (each chunk Morton-sorted on its own, so its
+    waves hold other points).  A bumpy sphere with points inside it: some waves hand their queries
+    over to the per-query kernel in both runs (a leaf budget of 256 leaves)."""
+    prev = S.set_heavy_leaves(256)
+    try:
+        _persistent_vs_chunks(mode)
+    finally:
+        S.set_heavy_leaves(prev)
+
+
+def _persistent_vs_chunks(mode):
+    m = mg.displaced_icosphere(20000, seed=5)
+    rm = S.ResidentMesh.from_mesh(m)
+    rng = np.random.default_rng(11)
+    n = 600_000
+    P = rng.uniform(-1.25, 1.25, (n, 3)).astype(np.float32)
+    x, y, z = _dev(*(np.ascontiguousarray(P[:, d]) for d in range(3)))
+    init = np.full(n, 0.8, np.float32)  # EDF min-accumulates into the seed
+    o = S.DeviceArray.from_host(init)
+    t = S.DeviceArray.from_host(np.full(n, -9, np.int32))
+    rm.query_points(mode, x, y, z, o, t, flags=S.FLAG_FIXUP)
+    q_all, err = rm.last_stats()
+    assert err == 0
+    want_o, want_t = o.download(), t.download()
+    got_o, got_t = np.empty_like(want_o), np.empty_like(want_t)
+    q_chunks = 0
+    for c0 in range(0, n, 60_000):
+        sl = slice(c0, min(n, c0 + 60_000))
+        cx, cy, cz = _dev(*(np.ascontiguousarray(P[sl, d]) for d in range(3)))
+        co = S.DeviceArray.from_host(init[sl])
+        ct = S.DeviceArray.from_host(np.full(sl.stop - sl.start, -9, np.int32))
+        rm.query_points(mode, cx, cy, cz, co, ct, flags=S.FLAG_FIXUP)
+        q, err = rm.last_stats()
+        assert err == 0
+        q_chunks += q
+        got_o[sl], got_t[sl] = co.download(), ct.download()
+    assert np.array_equal(want_o, got_o) and np.array_equal(want_t, got_t)
+    assert q_all > 0 and q_chunks > 0  # the per-query kernel ran in both
+    if mode == S.MODE_SDF:
+        assert np.all(want_t >= 0)
+
+
 def test_cell_validate_environment(capfd, monkeypatch):
     """SSDF_CELL_VALIDATE=1 (serial/ssdf.hpp:991-1010): the C symbols check their result against the
     exact FP64 argmin of the same inputs, print the maximum and the norm of the relative differences
