@@ -8,16 +8,22 @@
 // are re-tested against each lane's current bound before the node is fetched again.  A leaf
 // reference names a run of consecutive leaves (subtrees of at most kMaxRun leaves are never
 // visited); the whole wave evaluates each triangle (point_triangle_dist_sq,
-// serial/ssdf.hpp:50-121, in FP32).
+// serial/ssdf.hpp:50-121, in FP32), or skips it when its plane lies beyond every lane's bound.
+// A query of more waves than the GPU holds at once runs as one resident set of blocks that take
+// their waves from per-XCD queues (WaveSource).
 //
 // FIX (FP32 + FP64 fixup): pruning uses a widened bound, band(d) = (d (1 + tau) + e)^2 with e an
 // FP32 error bound of a distance, and each lane lists the triangles inside that band.  After
 // traversal, a lane whose band holds one triangle is final; two or more are re-evaluated in FP64
 // (same formula, contraction off -> the oracle's bits) and the lowest index wins exact ties; a lane
-// whose list overflowed is queued for k_query_exact (FP64 traversal bounded by the lane's band).
+// whose list overflowed is queued for the per-query kernel.
+//
+// k_coop: the queue's entries (those overflowed lists, and the queries of packet waves past their
+// leaf budget), one wave per query: the exact FP64 search below the entry's FP32 band (plain mode:
+// the FP32 search below its bound).
 //
 // k_query_exact: one lane per query, FP64 box and triangle tests, private stack; exact argmin with
-// the canonical tie-break.  Used for FP64 meshes and for the fixup queue.
+// the canonical tie-break.  Used for FP64 meshes.
 #include "query.hpp"
 #include "radix.hpp"
 
@@ -501,9 +507,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
 }
 
 template <int STACK, int QSRC, int OUT, bool FIX>
-// 8 waves per SIMD (64 VGPRs): FIX spills 17 VGPRs and 32 SGPRs, all reloaded in the epilogue after
-// the walk (the FP64 candidate evaluation), none in the traversal loops, and still gains 8% on the
-// 200 k-triangle torus at 256^3 over its unconstrained 78 VGPRs and 6 waves.
+// 8 waves per SIMD (64 VGPRs): FIX spills about 60 VGPRs (the grid and point kernels), stored and
+// reloaded once per wave around the FP64 candidate evaluation and the hand-over, none in the
+// traversal loops (descent, leaf, pop).  Before the persistent launch it spilled 17 and still
+// gained 8% on the 200 k-triangle torus at 256^3 over its unconstrained 78 VGPRs and 6 waves.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_query_packet(const Node *__restrict__ nodes, const TriP *__restrict__ tris,
                                                       QueryArgs a) {
     __shared__ int cand[FIX ? kCand : 1][256];
@@ -921,22 +928,13 @@ static void launch_coop(const QueryArgs &a, bool fix, int nlev, hipStream_t s) {
         hipLaunchKernelGGL((k_coop<QSRC, OUT, false>), dim3(kCoopBlocks), dim3(64 * w), lds, s, a, nlev);
 }
 
-// QUEUE: the items are the fixup queue's entries (query id, FP32 band); else every query.
-template <typename G, int QSRC, int OUT, bool QUEUE>
+// Every query of the call, one lane each (grid-stride).
+template <typename G, int QSRC, int OUT>
 __global__ __launch_bounds__(64) void k_query_exact(QueryArgs a) {
-    const unsigned n_items = QUEUE ? min(*a.fix_count, a.fix_cap) : 0u;
-    const long long total = QUEUE ? (long long)n_items
-                                  : (QSRC == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz);
+    const long long total = QSRC == QSRC_POINTS ? a.npoints : (long long)a.nx * a.ny * a.nz;
     for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
-         item += (long long)gridDim.x * blockDim.x) {
-        if (QUEUE)
-            exact_one<G, QSRC, OUT>(a,
-                                    (long long)(((unsigned long long)a.fix_list[4 * item + 3] << 32) |
-                                                a.fix_list[4 * item]),
-                                    (double)__uint_as_float(a.fix_list[4 * item + 1]));
-        else
-            exact_one<G, QSRC, OUT>(a, item, __builtin_inf());
-    }
+         item += (long long)gridDim.x * blockDim.x)
+        exact_one<G, QSRC, OUT>(a, item, __builtin_inf());
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1190,16 +1188,16 @@ static int dispatch_packet_out(QueryArgs &a, int out, bool fix, unsigned nblocks
     return SSDF_OK;
 }
 
-template <typename G, int QSRC, bool QUEUE>
+template <typename G, int QSRC>
 static int dispatch_exact_out(QueryArgs &a, int out, unsigned nblocks, hipStream_t s) {
     if (out == OUT_EDF)
-        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_EDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_EDF>), dim3(nblocks), dim3(64), 0, s, a);
     else if (out == OUT_SDF)
-        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SDF>), dim3(nblocks), dim3(64), 0, s, a);
     else if (out == OUT_SQEDF)
-        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SQEDF, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_SQEDF>), dim3(nblocks), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_CLOSEST, QUEUE>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_query_exact<G, QSRC, OUT_CLOSEST>), dim3(nblocks), dim3(64), 0, s, a);
     return SSDF_OK;
 }
 
@@ -1226,8 +1224,8 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
             return SSDF_ERR_UNSUPPORTED;
         }
         const unsigned nb = (unsigned)std::min<long long>((nq + 63) / 64, 1 << 20);
-        const int rc = qsrc == QSRC_POINTS ? dispatch_exact_out<double, QSRC_POINTS, false>(a, out, nb, stream)
-                                           : dispatch_exact_out<double, QSRC_GRID, false>(a, out, nb, stream);
+        const int rc = qsrc == QSRC_POINTS ? dispatch_exact_out<double, QSRC_POINTS>(a, out, nb, stream)
+                                           : dispatch_exact_out<double, QSRC_GRID>(a, out, nb, stream);
         SSDF_HIP_TRY(hipGetLastError());
         return rc;
     }
