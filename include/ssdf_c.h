@@ -262,7 +262,11 @@ int ssdf_batch_destroy(ssdf_batch_t batch);
 /* Rebuilds every mesh's LBVH (and normals) from the batch's vertex arena, in place. */
 int ssdf_batch_rebuild(ssdf_batch_t batch, int with_normals, void *stream);
 /* Mesh i's grid: dims[3i..3i+2] nodes over boxes[6i..6i+5] (min xyz, max xyz), x fastest, its
- * results at out_d/out_tri + out_offsets[i].  mode EDF (min-accumulate) or SDF (overwrite). */
+ * results at out_d/out_tri + out_offsets[i].  mode EDF (min-accumulate) or SDF (overwrite).
+ * The persistent launch hands out the work units (one wave's 64 queries of one mesh) costliest
+ * first, as the batch's previous query of the same plan measured them (the first query of a plan:
+ * natural order); results do not depend on the order.  Calls on one batch must not overlap on
+ * different streams (they share its work queues). */
 int ssdf_batch_query_grids(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, const double *boxes,
                            const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream);
 /* Mesh i's points: [point_offsets[i], point_offsets[i+1]) of x/y/z, results at the same positions. */
@@ -291,7 +295,9 @@ int ssdf_batch_autotune(ssdf_batch_t batch, int mode, const ptrdiff_t *dims, con
                         const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream,
                         int *best_blocks_per_cu, int *best_units_per_grab, float *best_ms);
 /* Captures [rebuild of every mesh +] the grid query into a hipGraph (stream must be a created
- * stream); ssdf_batch_replay launches it.  The graph holds kernel launches only. */
+ * stream); ssdf_batch_replay launches it.  The graph holds kernel launches only, and owns its
+ * descriptors, fixup queue and unit order, so eager queries of other plans may run between
+ * replays. */
 int ssdf_batch_capture(ssdf_batch_t batch, int rebuild, int mode, const ptrdiff_t *dims, const double *boxes,
                        const ptrdiff_t *out_offsets, float *out_d, int *out_tri, unsigned flags, void *stream);
 int ssdf_batch_replay(ssdf_batch_t batch, void *stream);
