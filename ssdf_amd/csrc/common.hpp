@@ -63,16 +63,23 @@ struct alignas(16) TriF {
 };
 static_assert(sizeof(TriF) == 48, "TriF must be 48 bytes");
 
-// Triangle record of the FP32 traversal: the TriF header plus per-triangle constants of the
-// branch-free distance (tri_d2_fast), so that a wave reading it with scalar loads spends no VALU on
-// them: edges e0 = b-a, e1 = c-b, e2 = a-c; in-plane inward edge normals m_i = n x e_i; inverse
-// squared edge lengths; unit normal.  144 bytes, in leaf (Morton) order.
+// Triangle record of the FP32 traversal: the TriF header plus the triangle in a plane frame of its
+// own, so that the branch-free distance (tri_d2_fast) works in two dimensions and a wave reading
+// the record with scalar loads spends no VALU on setting it up.  The frame's origin o is the first
+// corner of the longest edge, U points along that edge, V lies in the plane towards the third
+// corner, and N = U x V is the unit normal (the orientation of (b - a) x (c - a)).  In (U, V)
+// coordinates the corners are (0, 0), (bx, 0) and (cx, cy), cy >= 0.  144 bytes, in leaf (Morton)
+// order.
 struct alignas(16) TriP {
     TriF f;
-    float e[9];
-    float m[9];
-    float inv[3];
-    float nh[3];
+    float o[3];
+    float nh[3];               // N
+    float ux[3], vx[3];        // U, V
+    float bx, ibx;             // second corner (bx, 0); 1 / bx (0 when bx = 0)
+    float e1x, e1y, f1x, f1y;  // edge e1 from (bx, 0) to (cx, cy); e1 / |e1|^2 (0 when e1 = 0)
+    float cx, cy, g2x, g2y;    // third corner c; c / |c|^2 (0 when c = 0)
+    float in0;                 // inside threshold of the side tests: 0, or +inf on a degenerate triangle
+    float pad;
 };
 static_assert(sizeof(TriP) == 144, "TriP must be 144 bytes");
 
@@ -211,55 +218,53 @@ __host__ __device__ __forceinline__ float box_d2_fast(const float px, const floa
     return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
 }
 
-// Branch-free FP32 squared point-triangle distance: the plane distance when the projection of p
-// falls inside (all three in-plane side tests non-negative), else the nearest of the three edge
-// segments.  Same quantity as the Voronoi-region routine pt_tri_d2 (serial/ssdf.hpp:50-121) to a
-// few ulps of the coordinates, with no per-lane branches or divisions (every lane of a wave runs the
-// same ~65 instructions); the exact FP64 routine re-decides near-ties.  Degenerate triangles carry
-// NaN side normals, so they take the segment branch.
+// Branch-free FP32 squared point-triangle distance in the triangle's plane frame: h^2 plus the
+// squared 2D distance from the projection (u, v) of p to the triangle, which is 0 when (u, v) falls
+// inside (all three side tests non-negative) and otherwise the nearest of the three edge segments.
+// Same quantity as the Voronoi-region routine pt_tri_d2 (serial/ssdf.hpp:50-121) to a few ulps of
+// the coordinates, with no per-lane branches or divisions (every lane of a wave runs the same ~40
+// instructions); the exact FP64 routine re-decides near-ties.  Degenerate triangles have an
+// infinite inside threshold, so they take the segment branch.
 //
-// It is split in two so that a wave can stop after the first part: tri_plane gives p - a and the
-// signed plane distance h = n.(p - a) (unit normal n; 0 on triangles flagged degenerate), and |h|
-// bounds the distance from below (plane_bound below); tri_d2_rest finishes from there.
+// It is split in two so that a wave can stop after the first part: tri_plane gives p - o and the
+// signed plane distance h = N.(p - o), and |h| bounds the distance from below (plane_bound below);
+// tri_d2_rest finishes from there.
 struct TriPlane {
     float pa0, pa1, pa2, h;
 };
 __host__ __device__ __forceinline__ TriPlane tri_plane(const float px, const float py, const float pz, const TriP &t) {
     TriPlane r;
-    r.pa0 = px - t.f.a[0], r.pa1 = py - t.f.a[1], r.pa2 = pz - t.f.a[2];
+    r.pa0 = px - t.o[0], r.pa1 = py - t.o[1], r.pa2 = pz - t.o[2];
     r.h = __builtin_fmaf(t.nh[0], r.pa0, __builtin_fmaf(t.nh[1], r.pa1, t.nh[2] * r.pa2));
     return r;
 }
 
+__host__ __device__ __forceinline__ float clamp01(const float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
 __host__ __device__ __forceinline__ float tri_d2_rest(const float px, const float py, const float pz, const TriP &t,
                                                       const TriPlane &pl) {
     const float pa0 = pl.pa0, pa1 = pl.pa1, pa2 = pl.pa2, h = pl.h;
-    const float pb0 = px - t.f.b[0], pb1 = py - t.f.b[1], pb2 = pz - t.f.b[2];
-    const float pc0 = px - t.f.c[0], pc1 = py - t.f.c[1], pc2 = pz - t.f.c[2];
-    const float s0 = __builtin_fmaf(t.m[0], pa0, __builtin_fmaf(t.m[1], pa1, t.m[2] * pa2));
-    const float s1 = __builtin_fmaf(t.m[3], pb0, __builtin_fmaf(t.m[4], pb1, t.m[5] * pb2));
-    const float s2 = __builtin_fmaf(t.m[6], pc0, __builtin_fmaf(t.m[7], pc1, t.m[8] * pc2));
-    // edge segments: t = clamp(e.p / |e|^2, 0, 1), r = p - t e
-    float u0 = __builtin_fmaf(t.e[0], pa0, __builtin_fmaf(t.e[1], pa1, t.e[2] * pa2)) * t.inv[0];
-    float u1 = __builtin_fmaf(t.e[3], pb0, __builtin_fmaf(t.e[4], pb1, t.e[5] * pb2)) * t.inv[1];
-    float u2 = __builtin_fmaf(t.e[6], pc0, __builtin_fmaf(t.e[7], pc1, t.e[8] * pc2)) * t.inv[2];
-    u0 = fminf(fmaxf(u0, 0.0f), 1.0f);
-    u1 = fminf(fmaxf(u1, 0.0f), 1.0f);
-    u2 = fminf(fmaxf(u2, 0.0f), 1.0f);
-    const float r00 = __builtin_fmaf(-u0, t.e[0], pa0), r01 = __builtin_fmaf(-u0, t.e[1], pa1),
-                r02 = __builtin_fmaf(-u0, t.e[2], pa2);
-    const float r10 = __builtin_fmaf(-u1, t.e[3], pb0), r11 = __builtin_fmaf(-u1, t.e[4], pb1),
-                r12 = __builtin_fmaf(-u1, t.e[5], pb2);
-    const float r20 = __builtin_fmaf(-u2, t.e[6], pc0), r21 = __builtin_fmaf(-u2, t.e[7], pc1),
-                r22 = __builtin_fmaf(-u2, t.e[8], pc2);
-    const float q0 = __builtin_fmaf(r00, r00, __builtin_fmaf(r01, r01, r02 * r02));
-    const float q1 = __builtin_fmaf(r10, r10, __builtin_fmaf(r11, r11, r12 * r12));
-    const float q2 = __builtin_fmaf(r20, r20, __builtin_fmaf(r21, r21, r22 * r22));
-    const bool inside = (s0 >= 0.0f) & (s1 >= 0.0f) & (s2 >= 0.0f);
-    // bit select, so that both branches stay straight-line code for the whole wave
-    const int msk = -(int)inside;
-    const int dp = __builtin_bit_cast(int, h * h), ds = __builtin_bit_cast(int, fminf(q0, fminf(q1, q2)));
-    return __builtin_bit_cast(float, (dp & msk) | (ds & ~msk));
+    const float u = __builtin_fmaf(t.ux[0], pa0, __builtin_fmaf(t.ux[1], pa1, t.ux[2] * pa2));
+    const float v = __builtin_fmaf(t.vx[0], pa0, __builtin_fmaf(t.vx[1], pa1, t.vx[2] * pa2));
+    // segments s(t) = start + t e, t = clamp((p - start).e / |e|^2, 0, 1); r = p - s(t)
+    // from (0, 0) to (bx, 0)
+    const float t0 = clamp01(u * t.ibx);
+    const float r0 = __builtin_fmaf(-t0, t.bx, u);
+    const float q0 = __builtin_fmaf(r0, r0, v * v);
+    // from (bx, 0) to (cx, cy); side test cross(e1, p - b)
+    const float d1 = u - t.bx;
+    const float s1 = __builtin_fmaf(t.e1x, v, -(t.e1y * d1));
+    const float t1 = clamp01(__builtin_fmaf(t.f1x, d1, t.f1y * v));
+    const float r1x = __builtin_fmaf(-t1, t.e1x, d1), r1y = __builtin_fmaf(-t1, t.e1y, v);
+    const float q1 = __builtin_fmaf(r1x, r1x, r1y * r1y);
+    // from (0, 0) to (cx, cy); side test cross(p, c) <= 0
+    const float s2 = __builtin_fmaf(t.cy, u, -(t.cx * v));
+    const float t2 = clamp01(__builtin_fmaf(t.g2x, u, t.g2y * v));
+    const float r2x = __builtin_fmaf(-t2, t.cx, u), r2y = __builtin_fmaf(-t2, t.cy, v);
+    const float q2 = __builtin_fmaf(r2x, r2x, r2y * r2y);
+    // side test of the first edge: v >= 0
+    const bool inside = fminf(v, fminf(s1, s2)) >= t.in0;
+    return __builtin_fmaf(h, h, inside ? 0.0f : fminf(q0, fminf(q1, q2)));
 }
 
 __host__ __device__ __forceinline__ float tri_d2_fast(const float px, const float py, const float pz, const TriP &t) {
@@ -268,52 +273,78 @@ __host__ __device__ __forceinline__ float tri_d2_fast(const float px, const floa
 
 // Plane bound of a pruning bound d2b (a squared distance, strict or inclusive): every triangle whose
 // tri_d2_fast value is at most d2b has |h| <= plane_bound(r, babs), where r is sqrt(d2b) to 1 ulp
-// or above and babs >= 2^-18 max(L, |p|) (L: the largest coordinate magnitude of the mesh).  Why:
-// sqrt(d_fast) is within 2^-19 max of the exact distance d, |h_exact| <= d, and the FP32 h is
-// within 18 * 2^-24 max of h_exact (the roundings of p - a and of the unit normal, three in the
-// FMA chain, over |p - a| <= 2 sqrt(3) max); 2 babs covers those terms, and the factor on r its
-// rounding.  r = NaN or -inf (no query, or one handed over) gives a bound that no |h| passes.
+// or above and babs >= 0.  Why: tri_d2_fast rounds h^2 + q with q >= 0 once, so
+// |h| <= sqrt(d_fast) (1 + 2^-24) <= r (1 + 2^-24), and the factor on r covers that and its own
+// rounding; the 2 babs (babs = 2^-18 max(L, |p|) in the kernels) is margin on top.  r = NaN or
+// -inf (no query, or one handed over) gives a bound that no |h| passes.
 __host__ __device__ __forceinline__ float plane_bound(const float r, const float babs) {
     return __builtin_fmaf(r, 1.0f + 0x1p-22f, 2.0f * babs);
 }
 
-// Fills a TriP from FP32 vertices: the constants are evaluated in FP64 from the FP32 values and
-// rounded once.  A sliver whose smallest height is at most 1e-6 of its longest edge
-// (|n| <= 1e-6 |e|^2, n = 2 x area) is flagged degenerate: NaN side normals send every query to the
-// segment branch of tri_d2_fast, whose error is then at most that height (a projection inside the
-// triangle lies within it of an edge).  Thicker triangles meet the FP32 error bound of the fixup
-// band (2^-18 of the coordinate scale); thinner unflagged ones do not: their FP32 side tests cannot
-// resolve the strip they cover, and points along their axis were taken as inside.
+// Fills a TriP from FP32 vertices: the frame and the 2D corners are evaluated in FP64 from the FP32
+// values and rounded once.  The frame is Gram-Schmidt on the longest edge: V is the part of the
+// third corner's offset perpendicular to U, so the three corners lie in the (U, V) plane to FP64
+// rounding even on slivers, whose cross product cancels.  A sliver whose smallest height is at most
+// 1e-6 of its longest edge (|n| <= 1e-6 |e|^2, n = 2 x area) is flagged degenerate: its infinite
+// inside threshold sends every query to the segment branch of tri_d2_fast, whose error is then at
+// most that height (a projection inside the triangle lies within it of an edge).  Thicker triangles
+// meet the FP32 error bound of the fixup band (2^-18 of the coordinate scale); thinner unflagged
+// ones do not: their FP32 side tests cannot resolve the strip they cover.
 __host__ __device__ inline void make_trip(TriP &tr, const float *va, const float *vb, const float *vc, int orig) {
     double v[3][3];
     for (int d = 0; d < 3; d++) {
         tr.f.a[d] = va[d], tr.f.b[d] = vb[d], tr.f.c[d] = vc[d];
         v[0][d] = va[d], v[1][d] = vb[d], v[2][d] = vc[d];
     }
-    double e[3][3], n[3];
-    for (int q = 0; q < 3; q++)
+    double e[3][3], n[3], l2[3];
+    for (int q = 0; q < 3; q++) {
         for (int d = 0; d < 3; d++) e[q][d] = v[(q + 1) % 3][d] - v[q][d];
+        l2[q] = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
+    }
     // n = (b - a) x (c - a) = e0 x (-e2)
     n[0] = e[0][2] * e[2][1] - e[0][1] * e[2][2];
     n[1] = e[0][0] * e[2][2] - e[0][2] * e[2][0];
     n[2] = e[0][1] * e[2][0] - e[0][0] * e[2][1];
     const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-    double emax = 0;
-    for (int q = 0; q < 3; q++) {
-        const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
-        emax = l2 > emax ? l2 : emax;
-    }
+    int o = 0;  // the longest edge runs from corner o to corner o + 1 (a cyclic relabelling keeps n)
+    for (int q = 1; q < 3; q++)
+        if (l2[q] > l2[o]) o = q;
+    const double emax = l2[o];
     const bool degenerate = !(emax > 0) || !(nn > 1e-12 * emax * emax);
-    const double inv_nl = degenerate ? 0.0 : 1.0 / sqrt(nn);
-    for (int q = 0; q < 3; q++) {
-        const double l2 = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
-        tr.inv[q] = l2 > 0 ? (float)(1.0 / l2) : 0.0f;
-        for (int d = 0; d < 3; d++) tr.e[3 * q + d] = (float)e[q][d];
-        const double m[3] = {n[1] * e[q][2] - n[2] * e[q][1], n[2] * e[q][0] - n[0] * e[q][2],
-                             n[0] * e[q][1] - n[1] * e[q][0]};
-        for (int d = 0; d < 3; d++) tr.m[3 * q + d] = degenerate ? __builtin_nanf("") : (float)(m[d] * inv_nl);
+    const double *p0 = v[o], *p2 = v[(o + 2) % 3];
+    const double el = sqrt(emax);
+    double U[3] = {1.0, 0.0, 0.0}, V[3], w[3];
+    if (el > 0)
+        for (int d = 0; d < 3; d++) U[d] = e[o][d] / el;
+    for (int d = 0; d < 3; d++) w[d] = p2[d] - p0[d];
+    const double wu = w[0] * U[0] + w[1] * U[1] + w[2] * U[2];
+    for (int d = 0; d < 3; d++) V[d] = w[d] - wu * U[d];
+    double vl = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    if (!(vl > 0)) {
+        // collinear corners: any unit vector perpendicular to U (U x the axis U is least along)
+        const int k = (fabs(U[0]) <= fabs(U[1]) && fabs(U[0]) <= fabs(U[2])) ? 0 : (fabs(U[1]) <= fabs(U[2]) ? 1 : 2);
+        const double ax[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+        V[0] = U[1] * ax[2] - U[2] * ax[1], V[1] = U[2] * ax[0] - U[0] * ax[2], V[2] = U[0] * ax[1] - U[1] * ax[0];
+        vl = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
     }
-    for (int d = 0; d < 3; d++) tr.nh[d] = (float)(n[d] * inv_nl);
+    for (int d = 0; d < 3; d++) V[d] /= vl;
+    const double N[3] = {U[1] * V[2] - U[2] * V[1], U[2] * V[0] - U[0] * V[2], U[0] * V[1] - U[1] * V[0]};
+    const double cx = wu, cy = w[0] * V[0] + w[1] * V[1] + w[2] * V[2];
+    const double e1x = cx - el, e1y = cy, l1 = e1x * e1x + e1y * e1y, lc = cx * cx + cy * cy;
+    for (int d = 0; d < 3; d++) {
+        tr.o[d] = (float)p0[d];
+        tr.nh[d] = (float)N[d];
+        tr.ux[d] = (float)U[d];
+        tr.vx[d] = (float)V[d];
+    }
+    tr.bx = (float)el;
+    tr.ibx = el > 0 ? (float)(1.0 / el) : 0.0f;
+    tr.e1x = (float)e1x, tr.e1y = (float)e1y;
+    tr.f1x = l1 > 0 ? (float)(e1x / l1) : 0.0f, tr.f1y = l1 > 0 ? (float)(e1y / l1) : 0.0f;
+    tr.cx = (float)cx, tr.cy = (float)cy;
+    tr.g2x = lc > 0 ? (float)(cx / lc) : 0.0f, tr.g2y = lc > 0 ? (float)(cy / lc) : 0.0f;
+    tr.in0 = degenerate ? SSDF_INF : 0.0f;
+    tr.pad = 0.0f;
     tr.f.orig = orig;
     tr.f.flags = degenerate ? 1 : 0;
     tr.f.pad = 0;

@@ -194,12 +194,9 @@ __device__ __forceinline__ void packet_write(const QueryArgs &a, const long long
 // compiler sinks the loads of the fields that only the full distance reads below the test's
 // branch, so that a triangle that passes waits for a second round of scalar loads.
 __device__ __forceinline__ void tri_loads_issued(const TriP &t) {
-    asm volatile("" ::"s"(t.f.b[0]), "s"(t.f.b[1]), "s"(t.f.b[2]), "s"(t.f.c[0]), "s"(t.f.c[1]), "s"(t.f.c[2]));
-    asm volatile("" ::"s"(t.e[0]), "s"(t.e[1]), "s"(t.e[2]), "s"(t.e[3]), "s"(t.e[4]), "s"(t.e[5]), "s"(t.e[6]),
-                 "s"(t.e[7]), "s"(t.e[8]));
-    asm volatile("" ::"s"(t.m[0]), "s"(t.m[1]), "s"(t.m[2]), "s"(t.m[3]), "s"(t.m[4]), "s"(t.m[5]), "s"(t.m[6]),
-                 "s"(t.m[7]), "s"(t.m[8]));
-    asm volatile("" ::"s"(t.inv[0]), "s"(t.inv[1]), "s"(t.inv[2]));
+    asm volatile("" ::"s"(t.ux[0]), "s"(t.ux[1]), "s"(t.ux[2]), "s"(t.vx[0]), "s"(t.vx[1]), "s"(t.vx[2]));
+    asm volatile("" ::"s"(t.bx), "s"(t.ibx), "s"(t.e1x), "s"(t.e1y), "s"(t.f1x), "s"(t.f1y));
+    asm volatile("" ::"s"(t.cx), "s"(t.cy), "s"(t.g2x), "s"(t.g2y), "s"(t.in0));
 }
 
 // One descent step's decision from the ballots of the node's two children (ml: lanes that want the

@@ -96,7 +96,9 @@ def test_tri_d2_fast_degenerate(exe, oracle):
     d, bx, fl = run(exe, cases)
     assert np.all(np.isfinite(d))
     h, _ = check_plane_skip(exe, cases, d)
-    assert np.all(h[fl == 1] == 0)  # no normal: never skipped by the plane test
+    assert np.all(fl[:-1] == 1) and fl[-1] == 1
+    # a plane through the corners: its distance never exceeds the FP32 distance (to 1 ulp)
+    assert np.all(np.abs(h) <= np.sqrt(d) * (1 + 2.0 ** -23))
     for (p, a, b, c), dd in zip(cases, d):
         p, a, b, c = (x.astype(np.float64) for x in (p, a, b, c))
         seg = []
