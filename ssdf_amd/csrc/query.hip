@@ -234,6 +234,36 @@ __device__ __forceinline__ void descent_vote(const uint64_t ml, const uint64_t m
         : "scc");
 }
 
+// descent_vote with the push folded in, for one-VGPR stacks: the other child's entry goes to lane
+// sp (the depth before the vote) of stackv by v_writelane, lane select m0 (a VOP3 reads one SGPR, and
+// the entry is in one), so that the depth is updated in place instead of through copies.
+__device__ __forceinline__ void descent_vote_push(const uint64_t ml, const uint64_t mr, const uint64_t mle, int &node,
+                                                  const int left, const int right, int &sp, int &stackv) {
+    uint64_t t, many;
+    int a, b, next, ent;
+    asm("s_mov_b32 m0, %[sp]\n\t"
+        "s_orn2_b64 %[t], %[mle], %[mr]\n\t"
+        "s_and_b64 %[t], %[t], %[ml]\n\t"
+        "s_bcnt1_i32_b64 %[a], %[t]\n\t"
+        "s_or_b64 %[many], %[ml], %[mr]\n\t"
+        "s_bcnt1_i32_b64 %[b], %[many]\n\t"
+        "s_lshl_b32 %[a], %[a], 1\n\t"
+        "s_cmp_ge_u32 %[a], %[b]\n\t"
+        "s_cselect_b32 %[next], %[left], %[right]\n\t"
+        "s_cselect_b64 %[t], %[mr], %[ml]\n\t"
+        "s_cselect_b32 %[a], 0x80000000, 0\n\t"
+        "s_or_b32 %[ent], %[node], %[a]\n\t"
+        "s_cmp_lg_u64 %[t], 0\n\t"
+        "s_addc_u32 %[sp], %[sp], 0\n\t"
+        "s_cmp_lg_u64 %[many], 0\n\t"
+        "s_cselect_b32 %[node], %[next], 0x80000000\n\t"
+        "v_writelane_b32 %[stk], %[ent], m0"
+        : [t] "=&s"(t), [many] "=&s"(many), [a] "=&s"(a), [b] "=&s"(b), [ent] "=&s"(ent), [next] "=&s"(next),
+          [sp] "+s"(sp), [node] "+s"(node), [stk] "+v"(stackv)
+        : [ml] "s"(ml), [mr] "s"(mr), [mle] "s"(mle), [left] "s"(left), [right] "s"(right)
+        : "scc", "m0");
+}
+
 // Stack entry i (parent | side << 31): the pending child's box and reference, read from the parent
 // record with scalar loads issued together (the reference is not left for after the box test).
 template <int STACK>
@@ -316,23 +346,28 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         // descend through internal nodes, nearest child first by majority vote; the decision and the
         // push are uniform selects, so a visit is straight-line code
         while (node >= 0) {
-            const Node nd = nodes[node];
+            // a 32-bit byte offset lets the scalar loads take it as their SGPR offset (node < 2^26)
+            const Node nd = *(const Node *)((const char *)nodes + ((unsigned)node << 6));
             const float dl = box_d2_fast(qx, qy, qz, nd.lmin, nd.lmax);
             const float dr = box_d2_fast(qx, qy, qz, nd.rmin, nd.rmax);
             const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
             const uint64_t mr = __ballot(FIX ? dr <= thr : dr < best);
             const uint64_t mle = __ballot(dl <= dr);
-            const int sp0 = sp;
-            int ent;
-            descent_vote(ml, mr, mle, node, nd.left, nd.right, sp, ent);
             // the other child's entry is written at the old depth whether pushed or not (above the
             // top it is overwritten before any pop reads it), by v_writelane from a scalar.  The
             // depth is at most this node's depth < STACK: the stack holds at most one entry per
             // level above the node, and the launch picks STACK from the tree depth.
-            if (STACK == 64 || sp0 < 64)
-                stackv = amdgcn_writelane(ent, sp0 & 63, stackv);
-            else
-                stackv_hi = amdgcn_writelane(ent, sp0 & 63, stackv_hi);
+            if (STACK == 64) {
+                descent_vote_push(ml, mr, mle, node, nd.left, nd.right, sp, stackv);
+            } else {
+                const int sp0 = sp;
+                int ent;
+                descent_vote(ml, mr, mle, node, nd.left, nd.right, sp, ent);
+                if (sp0 < 64)
+                    stackv = amdgcn_writelane(ent, sp0 & 63, stackv);
+                else
+                    stackv_hi = amdgcn_writelane(ent, sp0 & 63, stackv_hi);
+            }
         }
         overflow |= sp > STACK;
         // The pop that follows re-reads the top entry's box and child reference; those scalar loads
