@@ -452,8 +452,12 @@ int reorder_units(const UnitOrder &o, unsigned U, hipStream_t st) {
     return SSDF_OK;
 }
 
+// Stack width of a batch's traversal: 64 entries (32-bit node offsets) while every tree is at most
+// 64 deep and has fewer than kNarrowNodes internal nodes, else 128 (64-bit offsets).
 int stack_of(const ssdf_batch *b) {
-    if (b->maxdepth <= 64) return 64;
+    long long maxsel = 0;
+    for (const auto &m : b->meshes) maxsel = std::max<long long>(maxsel, m.nsel);
+    if (b->maxdepth <= 64 && maxsel - 1 < kNarrowNodes) return 64;
     if (b->maxdepth <= 128) return 128;
     return 0;
 }

@@ -264,6 +264,15 @@ __device__ __forceinline__ void descent_vote_push(const uint64_t ml, const uint6
         : "scc", "m0");
 }
 
+// Byte b of the node array.  STACK == 64 launches serve trees of fewer than kNarrowNodes internal
+// nodes (the launch picks STACK from the depth and the size), whose byte offsets fit 32 bits: the
+// scalar loads then take the offset as their SGPR offset (one shift instead of a 64-bit multiply-add);
+// STACK == 128 launches index in 64 bits.
+template <int STACK, typename T>
+__device__ __forceinline__ const T &node_bytes(const Node *__restrict__ nodes, const unsigned long long b) {
+    return STACK == 64 ? *(const T *)((const char *)nodes + (unsigned)b) : *(const T *)((const char *)nodes + b);
+}
+
 // Stack entry i (parent | side << 31): the pending child's box and reference, read from the parent
 // record with scalar loads issued together (the reference is not left for after the box test).
 template <int STACK>
@@ -271,11 +280,18 @@ __device__ __forceinline__ void stack_entry_box(const Node *__restrict__ nodes, 
                                                 const int i, float *mn, float *mx, int &child) {
     const int e = (STACK == 64 || i < 64) ? __builtin_amdgcn_readlane(stackv, i & 63)
                                           : __builtin_amdgcn_readlane(stackv_hi, i & 63);
-    const int par = e & 0x7fffffff, side = (int)((unsigned)e >> 31);
-    const float *pb = (const float *)(nodes + par) + 6 * side;
+    const unsigned side = (unsigned)e >> 31;
+    // the record's byte offset: 32-bit e << 6 drops the side bit (STACK == 64 trees have fewer than
+    // 2^26 internal nodes), 64-bit otherwise
+    const unsigned long long rec =
+        STACK == 64 ? (unsigned long long)((unsigned)e << 6) : (unsigned long long)((unsigned)e & 0x7fffffffu) << 6;
+    struct Box6 {
+        float v[6];
+    };
+    const Box6 bx = node_bytes<STACK, Box6>(nodes, STACK == 64 ? (unsigned)rec + (side ? 24u : 0u) : rec + 24 * side);
 #pragma unroll
-    for (int d = 0; d < 3; d++) mn[d] = pb[d], mx[d] = pb[3 + d];
-    child = uniform_i(((const int *)(nodes + par))[12 + side]);
+    for (int d = 0; d < 3; d++) mn[d] = bx.v[d], mx[d] = bx.v[3 + d];
+    child = uniform_i(node_bytes<STACK, int>(nodes, STACK == 64 ? (unsigned)rec + (side ? 52u : 48u) : rec + 48 + 4 * side));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -346,8 +362,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         // descend through internal nodes, nearest child first by majority vote; the decision and the
         // push are uniform selects, so a visit is straight-line code
         while (node >= 0) {
-            // a 32-bit byte offset lets the scalar loads take it as their SGPR offset (node < 2^26)
-            const Node nd = *(const Node *)((const char *)nodes + ((unsigned)node << 6));
+            const Node nd = node_bytes<STACK, Node>(nodes, (unsigned long long)node << 6);
             const float dl = box_d2_fast(qx, qy, qz, nd.lmin, nd.lmax);
             const float dr = box_d2_fast(qx, qy, qz, nd.rmin, nd.rmax);
             const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
@@ -379,9 +394,10 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         if (node != kNone) {
           // a run of leaves (one, or the leaves of a subtree too small to visit)
           const int k0 = leaf_first(node), nlv = leaf_count(node);
+          const TriP *__restrict__ run = tris + k0;  // the run's records by 32-bit offsets from here
           for (int j = 0; j < nlv; j++) {
             const int k = k0 + j;
-            const TriP tr = tris[k];
+            const TriP tr = *(const TriP *)((const char *)run + (unsigned)j * (unsigned)sizeof(TriP));
             const TriPlane pl = tri_plane(qx, qy, qz, tr);
             tri_loads_issued(tr);
             if (__ballot(fabsf(pl.h) <= rb) == 0) continue;
@@ -1289,7 +1305,7 @@ int launch_query(const DevMesh &m, QueryArgs &a, int qsrc, int out, unsigned fla
     a.persistent = nblocks > resident && a.wqueue ? 1u : 0u;
     if (a.persistent) nblocks = resident;
     int rc;
-    if (depth <= 64) {
+    if (depth <= 64 && m.nsel - 1 < kNarrowNodes) {
         rc = qsrc == QSRC_POINTS ? dispatch_packet_out<64, QSRC_POINTS>(a, out, fix, nblocks, stream)
                                  : dispatch_packet_out<64, QSRC_GRID>(a, out, fix, nblocks, stream);
     } else if (depth <= 128) {

@@ -179,6 +179,9 @@ struct BatchArgs {
 
 // Waves per block of the packet and winding-number kernels.
 constexpr unsigned kWavesPerBlock = 4;
+// Internal nodes below which a tree's node records are addressed by 32-bit byte offsets (the
+// STACK == 64 traversal); larger trees take the STACK == 128 one, which indexes in 64 bits.
+constexpr long long kNarrowNodes = 1ll << 26;
 
 // The waves a block of a packet or winding-number launch answers: its own 4 (one block per 4
 // waves, block ids remapped over the XCDs), or, in a persistent launch, one at a time from the
