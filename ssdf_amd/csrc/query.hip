@@ -340,7 +340,7 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
     int bk = -1, ncand = 0;
     float bestd = SSDF_INF;  // FIX: FP32 best distance found
     // Plane bound of the lane's pruning bound (plane_bound): a triangle whose plane lies farther than
-    // this from every lane's query is skipped by the whole wave after 7 of its ~65 instructions.
+    // this from every lane's query is skipped by the whole wave after 7 of its ~40 instructions.
     float rb = valid ? plane_bound(__builtin_amdgcn_sqrtf(init2), babs) : -SSDF_INF;
 
 
