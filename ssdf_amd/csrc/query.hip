@@ -363,6 +363,8 @@ __device__ __forceinline__ void packet_wave(const Node *__restrict__ nodes, cons
         // push are uniform selects, so a visit is straight-line code
         while (node >= 0) {
             const Node nd = node_bytes<STACK, Node>(nodes, (unsigned long long)node << 6);
+            // the whole 64-byte record in one scalar load (first and last are not read by the walk)
+            asm volatile("" ::"s"(nd.first), "s"(nd.last));
             const float dl = box_d2_fast(qx, qy, qz, nd.lmin, nd.lmax);
             const float dr = box_d2_fast(qx, qy, qz, nd.rmin, nd.rmax);
             const uint64_t ml = __ballot(FIX ? dl <= thr : dl < best);
