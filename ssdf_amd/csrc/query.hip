@@ -192,11 +192,12 @@ __device__ __forceinline__ void packet_write(const QueryArgs &a, const long long
 
 // Keeps the scalar loads of a whole triangle record ahead of the plane test: left alone, the
 // compiler sinks the loads of the fields that only the full distance reads below the test's
-// branch, so that a triangle that passes waits for a second round of scalar loads.
+// branch, so that a triangle that passes waits for a second round of scalar loads.  The pad word
+// is named too, so that the record's 96 bytes after the header arrive in two loads, not three.
 __device__ __forceinline__ void tri_loads_issued(const TriP &t) {
     asm volatile("" ::"s"(t.ux[0]), "s"(t.ux[1]), "s"(t.ux[2]), "s"(t.vx[0]), "s"(t.vx[1]), "s"(t.vx[2]));
     asm volatile("" ::"s"(t.bx), "s"(t.ibx), "s"(t.e1x), "s"(t.e1y), "s"(t.f1x), "s"(t.f1y));
-    asm volatile("" ::"s"(t.cx), "s"(t.cy), "s"(t.g2x), "s"(t.g2y), "s"(t.in0));
+    asm volatile("" ::"s"(t.cx), "s"(t.cy), "s"(t.g2x), "s"(t.g2y), "s"(t.in0), "s"(t.pad));
 }
 
 // One descent step's decision from the ballots of the node's two children (ml: lanes that want the
